@@ -1,0 +1,188 @@
+/*
+ * pis_hip.h -- C ABI of the MI355X (gfx950) Lennard-Jones force and velocity-Verlet
+ * engine.  Plain pointers and sizes only; no C++ or framework types cross this line.
+ *
+ * Reference interfaces replaced (vivekadishankara/pis, Rust):
+ *   trait PotentialManager                  src/potentials/potential.rs:12-136
+ *     fn compute_potential(&self, &mut Atoms) -> f64   potential.rs:13
+ *     fn verlet_step_nve(&self, &mut Atoms, dt) -> f64  potential.rs:15-33
+ *   LJVOffsetManager (the shipped manager)   src/potentials/lennard_jones.rs:182-244
+ *   manager construction from pair_coeff     src/system.rs:121-165, readers/input_file/commands.rs:147-384
+ *   Atoms / SimulationBox state              src/atoms/new.rs:9-17, src/simulation_box.rs:5-15
+ *   thermo reductions                        src/atoms/properties.rs:17-65
+ *   PisError                                 src/errors.rs:5-100
+ *
+ * Conventions (identical to the reference's in-memory layout, so a Rust shim can pass
+ * Matrix3xX::as_ptr() directly):
+ *   - positions / velocities / forces: AoS xyz, 3*n_atoms doubles, original atom order;
+ *   - box matrices h and h_inv: 3x3 column-major (nalgebra), box vectors as columns;
+ *     h_inv is produced by the caller exactly as SimulationBox::new does (the engine
+ *     never recomputes it, so minimum image and wrap see the caller's bits);
+ *   - atom types are 1-based (properties.rs:10);
+ *   - errors: every entry point returns pis_status; pis_last_error() gives the message
+ *     of the last failure on the calling thread.  No exception crosses the ABI.
+ *
+ * Threading: a context is not re-entrant; separate contexts may be used from separate
+ * threads.  All device work of a context runs on one HIP stream owned by the context.
+ */
+#ifndef PIS_HIP_H
+#define PIS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIS_ABI_VERSION 1
+
+/* Status codes.  1-4 map to PisError variants (errors.rs:71-99). */
+typedef enum pis_status {
+    PIS_OK = 0,
+    PIS_ERR_NO_ATOMS = 1,          /* PisError::NoAtomsDefined (errors.rs:71-72) */
+    PIS_ERR_MISSING_POTENTIAL = 2, /* PisError::PotentialNotInitialized / MissingPotential */
+    PIS_ERR_NAN_ENERGY = 3,        /* PisError::NaNEnergy (errors.rs:90-91) */
+    PIS_ERR_INVALID_ATOM_TYPE = 4, /* PisError::InvalidAtomType (errors.rs:94-95) */
+    PIS_ERR_INVALID_ARGUMENT = 5,
+    PIS_ERR_UNSUPPORTED = 6, /* triclinic or non-periodic box, thin box with many atoms */
+    PIS_ERR_HIP = 7,
+    PIS_ERR_OUT_OF_MEMORY = 8,
+    PIS_ERR_NO_DEVICE = 9,
+    PIS_ERR_CAPACITY = 10, /* neighbour/staging capacity still exceeded after regrowth */
+    PIS_ERR_NOT_FOUND = 11
+} pis_status;
+
+typedef enum pis_precision {
+    PIS_PRECISION_FP64 = 0,  /* every operation in FP64 (the parity path) */
+    PIS_PRECISION_MIXED = 1  /* FP32 pair arithmetic; positions, velocities, minimum
+                                image, wrap, energy and virial sums in FP64 */
+} pis_precision;
+
+/* One pair_coeff line (system.rs:136-163): LennardJones::new(eps, sigma, rcut, shift).
+ * (type_i, type_j) is stored under (min, max) (potential.rs:181-192). */
+typedef struct pis_pair_coeff {
+    uint32_t type_i;
+    uint32_t type_j;
+    double epsilon;
+    double sigma;
+    double rcut;
+} pis_pair_coeff;
+
+typedef struct pis_system_desc {
+    uint64_t n_atoms;
+    uint32_t n_types;
+    uint32_t n_pairs;
+    const uint32_t* type_ids; /* n_atoms, 1-based */
+    const double* masses;     /* n_types, per type (Atoms::masses, new.rs:12) */
+    const pis_pair_coeff* pairs;
+    double h[9];
+    double h_inv[9];
+    uint8_t pbc[3];       /* all three must be 1 (from_lammps_data always sets them) */
+    uint8_t shift;        /* LennardJones::shift; the shipped path always sets 1 */
+    uint8_t precision;    /* pis_precision */
+    uint8_t reserved[3];
+    double skin;          /* Verlet skin; list cutoff is rcut_ij + skin, rebuilt when an atom
+                             moved more than skin/2 since the last build */
+    double kb;            /* Boltzmann constant used for temperature (constants.rs:3 is
+                             0.0083144621; reduced units use 1.0) */
+    uint32_t neighbor_capacity; /* 0 = automatic; otherwise entries per atom */
+    uint32_t flags;       /* reserved, must be 0 */
+} pis_system_desc;
+
+/* Per-step thermo record (Simulation::output, simulation.rs:79-86). */
+typedef struct pis_thermo {
+    double pe;          /* PotentialManager return value: sum of shifted pair energies */
+    double ke;          /* kinetic_energy (properties.rs:17-23) */
+    double etot;        /* NVE Hamiltonian pe + ke (simulation.rs:98-100) */
+    double temperature; /* 2 ke / (3 N kb) (properties.rs:28-30) */
+    double pressure;    /* (2 ke + sum_i r_i.F_i) / (3 V) (properties.rs:61-65) */
+    double virial_abs;  /* sum_i r_i . F_i, the reference's absolute-position virial */
+    double virial_pair; /* sum_{i<j} r_ij . F_ij, the periodic pair virial */
+} pis_thermo;
+
+/* Host-side plan of a system (no device needed): what the engine will build. */
+typedef struct pis_plan_info {
+    int32_t ncell[3];       /* cell grid, cell edge >= max rcut + skin */
+    int32_t tile[3];        /* cells per tile (last tile per dim may be smaller) */
+    int32_t ntile[3];
+    int32_t mode;           /* 0 tile-relative staging, 1 staged raw + per-pair minimum
+                               image, 2 whole system staged (thin boxes) */
+    int32_t neighbor_capacity;
+    int32_t stage_capacity; /* atoms a tile may stage in LDS */
+    double r2_list_max;     /* largest r^2 whose correctly rounded sqrt <= max rcut + skin */
+    double density;
+} pis_plan_info;
+
+typedef struct pis_stats {
+    uint64_t steps;
+    uint64_t rebuilds;
+    uint64_t force_launches;
+    double force_kernel_ms; /* summed HIP-event time of force launches while timing is on */
+    uint64_t timed_force_launches;
+    int32_t neighbor_capacity;
+    int32_t max_neighbors; /* largest per-atom list seen at the last build */
+} pis_stats;
+
+typedef struct pis_ctx pis_ctx;
+
+int pis_abi_version(void);
+const char* pis_last_error(void);
+const char* pis_status_string(pis_status s);
+pis_status pis_device_count(int32_t* count);
+
+/* Host-only: validate a descriptor and report the grid/tile/capacity plan. */
+pis_status pis_plan_system(const pis_system_desc* desc, pis_plan_info* out);
+
+/* Autotuner hook (no reference counterpart): cells per tile used by contexts created
+ * afterwards, {tx, ty, tz}; NULL or zeros restore the built-in heuristic.  Process-wide. */
+pis_status pis_set_tile_override(const int32_t* tile);
+
+/* Single-system context: the GPU LJVOffsetManager replacement. */
+pis_status pis_ctx_create(const pis_system_desc* desc, pis_ctx** out);
+void pis_ctx_destroy(pis_ctx* ctx);
+
+/* Upload state (original order, AoS xyz).  vel / force may be NULL (zeros). */
+pis_status pis_set_state(pis_ctx* ctx, const double* pos, const double* vel, const double* force);
+/* Download state in original order; any pointer may be NULL. */
+pis_status pis_get_state(pis_ctx* ctx, double* pos, double* vel, double* force);
+
+/* PotentialManager::compute_potential (lennard_jones.rs:187-243): forces of the current
+ * positions are ADDED into forces_inout (3n, may be NULL) as the reference accumulates
+ * into Atoms::forces; *pe receives the potential energy.  The device force array is
+ * overwritten with the computed forces (what verlet_step_nve uses as F(t)). */
+pis_status pis_compute_potential(pis_ctx* ctx, double* forces_inout, double* pe);
+
+/* nsteps of PotentialManager::verlet_step_nve (potential.rs:15-33) with state resident
+ * on the device.  per_step_out (nsteps records, may be NULL) receives thermo after each
+ * step; its pe is what verlet_step_nve returns. */
+pis_status pis_verlet_step_nve(pis_ctx* ctx, double dt, uint64_t nsteps, pis_thermo* per_step_out);
+
+/* Rebuild the neighbour list at the current positions and return it in original atom
+ * indices as CSR (offsets: n+1 entries; idx: capacity `cap`; *total gets the number of
+ * entries even when it exceeds cap).  Used by parity tests of the pair set. */
+pis_status pis_get_neighbor_list(pis_ctx* ctx, uint64_t* offsets, uint32_t* idx, uint64_t cap,
+                                 uint64_t* total);
+
+pis_status pis_get_stats(pis_ctx* ctx, pis_stats* out);
+/* Turn per-launch HIP-event timing of the force kernel on (1) or off (0); resets sums. */
+pis_status pis_set_kernel_timing(pis_ctx* ctx, int32_t enable);
+
+/* ---- Batched ensembles: many independent systems advanced by one launch per kernel
+ * per step (no reference counterpart beyond running PotentialManager per system). */
+pis_status pis_batch_create(const pis_system_desc* descs, uint32_t n_systems, pis_ctx** out);
+uint32_t pis_batch_size(const pis_ctx* ctx);
+pis_status pis_sys_set_state(pis_ctx* ctx, uint32_t sys, const double* pos, const double* vel,
+                             const double* force);
+pis_status pis_sys_get_state(pis_ctx* ctx, uint32_t sys, double* pos, double* vel, double* force);
+/* Initial forces for every system; pe_out[n_systems] may be NULL. */
+pis_status pis_batch_compute_potential(pis_ctx* ctx, double* pe_out);
+/* nsteps for every system, dt per system (dts[n_systems]); thermo_out is
+ * [nsteps][n_systems] records or NULL. */
+pis_status pis_batch_verlet_step_nve(pis_ctx* ctx, const double* dts, uint64_t nsteps,
+                                     pis_thermo* thermo_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PIS_HIP_H */

@@ -1,0 +1,121 @@
+/*
+ * pis_oracle.h -- CPU FP64 restatement of the reference's Lennard-Jones force and
+ * velocity-Verlet path (vivekadishankara/pis, Rust).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/, the smoke
+ * entry point and bench.py's CPU-baseline leg may load it.  The product path
+ * (pis_amd/libpis_hip.so) never links, loads or calls it.
+ *
+ * Built with gcc -O2 -ffp-contract=off (Rust never contracts a*b+c into an FMA, so
+ * neither may the restatement).  Every function cites the reference file:line it
+ * follows.  Matrices are 3x3 column-major doubles, like nalgebra's Matrix3
+ * (element (r,c) at index c*3+r); positions/velocities/forces are AoS xyz (3*n
+ * doubles), like nalgebra's Matrix3xX.  Atom types are 1-based.
+ *
+ * Parity status per function (what pins it):
+ *   orc_box_inverse      -- nalgebra 0.34.1 3x3 try_inverse, restated from its
+ *                           published source from memory; pinned only by
+ *                           h*h_inv = I and 1-ulp agreement with 1/L for diagonal
+ *                           boxes.  Exact bits for triclinic boxes: parity unpinned.
+ *   orc_gemv             -- nalgebra gemv column order; exact for orthorhombic boxes
+ *                           (off-diagonal terms are signed zeros), unpinned for tilt.
+ *   everything else      -- pinned by closed forms, brute force and invariants in
+ *                           tests/test_oracle.py (listed there per function).
+ */
+#ifndef PIS_ORACLE_H
+#define PIS_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    double h[9];    /* box vectors as columns */
+    double hinv[9];
+    int32_t pbc[3];
+} orc_box;
+
+/* Dense symmetric pair table, (ti-1)*ntypes + (tj-1); present==0 means the
+ * reference's HashMap lookup misses (potential.rs:181-192) and the pair is skipped. */
+typedef struct {
+    int32_t ntypes;
+    const int32_t* present;
+    const double* eps;
+    const double* sigma;
+    const double* rcut;
+    int32_t shift; /* LennardJones::shift (always true on the shipped path) */
+} orc_table;
+
+typedef struct {
+    int64_t n;
+    double* pos;          /* 3n */
+    double* vel;          /* 3n */
+    double* force;        /* 3n */
+    const int32_t* type;  /* n, 1-based */
+    const double* mass;   /* ntypes */
+    orc_box box;
+    orc_table table;
+} orc_sys;
+
+/* CSR neighbour list (full list, i != j). */
+typedef struct {
+    int64_t n;
+    int64_t* start; /* n+1 */
+    int64_t* idx;
+    int64_t cap;
+} orc_list;
+
+/* Manager selector for orc_compute / orc_verlet_step_nve. */
+enum {
+    ORC_MGR_OFFSET = 0,    /* LJVOffsetManager, the shipped one (lennard_jones.rs:182-244) */
+    ORC_MGR_BRUTE = 1,     /* LJManager O(N^2) (lennard_jones.rs:62-99) */
+    ORC_MGR_VERLET27 = 2,  /* LJVerletManager (lennard_jones.rs:103-178) */
+    ORC_MGR_FULLLIST = 3   /* LJVPBuildListManager (lennard_jones.rs:340-456) */
+};
+
+int orc_box_inverse(const double* h, double* hinv);
+void orc_box_from_lammps(double xlo, double xhi, double ylo, double yhi, double zlo,
+                         double zhi, double xy, double xz, double yz, orc_box* out);
+double orc_box_volume(const orc_box* b);
+void orc_gemv(const double* m, const double* x, double* y);
+void orc_min_image(const orc_box* b, double* d);
+void orc_wrap(const orc_box* b, double* r);
+void orc_divide_into_cells(const orc_box* b, double rcut, int64_t* ncell);
+int64_t orc_cell_of(const orc_box* b, const int64_t* ncell, const double* r);
+double orc_norm(const double* v);
+double orc_max_rcut(const orc_table* t);
+
+void orc_lj_pair(double eps, double sigma, double rcut, int32_t shift, const double* rij,
+                 double* u, double* f);
+
+double orc_compute(orc_sys* s, int32_t manager);
+double orc_compute_offset(orc_sys* s);
+double orc_compute_brute(orc_sys* s);
+double orc_compute_verlet27(orc_sys* s);
+
+int orc_build_list(const orc_sys* s, double rcut_add, orc_list* out);
+void orc_list_free(orc_list* l);
+int orc_brute_list(const orc_sys* s, double rcut_add, orc_list* out);
+double orc_compute_fulllist(orc_sys* s, const orc_list* l, int32_t check_rcut,
+                            double* pe_atom);
+
+double orc_verlet_step_nve(orc_sys* s, double dt, int32_t manager);
+double orc_kinetic_energy(const orc_sys* s);
+double orc_temperature(const orc_sys* s, double ke, double kb);
+double orc_virial_trace(const orc_sys* s);
+double orc_pressure(const orc_sys* s, double ke);
+
+/* Simulation::run NVE loop: initial compute, then nsteps of verlet_step_nve; per step
+ * writes PE, KE, H=PE+KE, T, P into thermo[5*k]. */
+int orc_run_nve(orc_sys* s, double dt, int64_t nsteps, int32_t manager, double kb,
+                double* thermo);
+
+/* Idealised parallel CPU baseline (not a restatement): full list + OpenMP gather. */
+int orc_ideal_run_nve(orc_sys* s, double dt, int64_t nsteps, double skin, int32_t nthreads,
+                      double* pe_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,99 @@
+// common.h -- device data layout shared by the gfx950 kernels and the host engine.
+//
+// One simulated system ("sys") owns:
+//   * two buffer sets of per-atom SoA state in cell-sorted order (double buffered so a
+//     rebuild can permute without a copy-back; the live set is chosen on the device by
+//     ctl[CTL_CUR] so captured launch sequences never need host decisions);
+//   * reference positions (r at the last list build) for the skin trigger;
+//   * a cell grid whose edge is >= max(rcut)+skin, cells numbered tile-major so that the
+//     atoms of one tile are contiguous;
+//   * a full neighbour list of u16 LDS byte offsets, 8 per uint4, laid out
+//     [group][atom] so a wavefront's loads of one group are one coalesced 1 KiB read.
+#pragma once
+#include <stdint.h>
+
+#include <hip/hip_runtime.h>
+
+namespace pis {
+
+constexpr int kTileThreads = 256;  // workgroup of the tile kernels (4 wave64)
+constexpr int kAtomThreads = 256;  // workgroup of the per-atom kernels
+constexpr int kMaxHalo = 256;      // halo-box cells a tile may stage
+constexpr int kMaxTypes = 4;
+
+// ctl[] slots (device ints per system)
+enum {
+    CTL_CUR = 0,       // live buffer set (0/1)
+    CTL_REBUILD = 1,   // rebuild requested for this step
+    CTL_OVF_NB = 2,    // neighbour capacity exceeded at the last build
+    CTL_OVF_STAGE = 3, // staging capacity exceeded
+    CTL_NAN = 4,       // non-finite position or energy seen
+    CTL_MAXNB = 5,     // largest list length at the last build
+    CTL_NREBUILD = 6,  // builds performed
+    CTL_STEP = 7,      // thermo record index
+    CTL_COUNT = 8
+};
+
+enum { MODE_TILE_REL = 0, MODE_RAW_MI = 1, MODE_WHOLE = 2 };
+
+struct PairParam {
+    double r2_force;  // largest r^2 with sqrt_rn(r^2) <= rcut (-1: pair absent)
+    double r2_list;   // largest r^2 with sqrt_rn(r^2) <= rcut + skin (-1: absent)
+    double sig2;      // sigma^2
+    double c4;        // 4 eps
+    double c24;       // 24 eps
+    double c48;       // 48 eps
+    double ucut;      // energy shift U(rcut) (0 without shift)
+    float sig2f, c4f, c24f, c48f, ucutf, pad;
+};
+
+struct State {
+    double *x, *y, *z;
+    double *vx, *vy, *vz;
+    double *fx, *fy, *fz;
+    uint8_t* type;  // 0-based
+    uint32_t* orig; // original atom index
+};
+
+struct ThermoDev {
+    double pe, ke, etot, temperature, pressure, virial_abs, virial_pair;
+};
+
+struct SysDev {
+    State st[2];
+    int32_t* ctl;
+    double *rx, *ry, *rz;  // positions at the last build (sorted order)
+    int32_t* cell_count;   // [ncell]
+    int32_t* cell_fill;    // [ncell]
+    int32_t* cell_start;   // [ncell+1], sorted (tile-major) cell ids
+    int32_t* nat2sorted;   // [ncell]: natural (cz*ny+cy)*nx+cx -> sorted id
+    int32_t* tile_cell0;   // [ntile+1]
+    int32_t* atom_cell;    // [n] sorted cell id of atom (live order) at binning
+    int32_t* scat_src;     // [n] scatter slot -> source atom
+    uint32_t* scat_key;    // [n] scatter slot -> orig index (sort key)
+    uint4* nbr;            // [(maxnb/8) * npad]
+    uint16_t* nnb;         // [npad]
+    double* partial;       // [ntile*4]: pe, ke, virial_abs, virial_pair
+    ThermoDev* thermo;     // [thermo_cap]
+    const PairParam* pp;   // [ntypes*ntypes]
+    const double* mass;    // [ntypes]
+    double L[3], Linv[3];  // h diagonal and h_inv diagonal (caller's bits)
+    double cellw[3];
+    double half_skin2;     // (skin/2)^2
+    double dt, dt2;
+    double kb, volume;
+    int32_t n, npad, ntypes, maxnb;
+    int32_t nc[3], ncell;
+    int32_t tile[3], nt[3], ntile;
+    int32_t mode;
+    int32_t thermo_cap;
+    int32_t pad0;
+};
+
+// Work items of batched launches.
+struct WorkItem {
+    int32_t sys;
+    int32_t idx;  // first atom of a 256-atom chunk, or tile id
+};
+
+}  // namespace pis

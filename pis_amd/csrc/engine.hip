@@ -1,0 +1,909 @@
+// engine.hip -- host side of the MI355X LJ engine: planning, device memory, the
+// device-resident step loop and the C ABI of include/pis_hip.h.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pis_hip.h"
+#include "common.h"
+#include "kernels.h"
+
+using namespace pis;
+
+// ---------------------------------------------------------------------------------
+// errors
+
+static thread_local std::string g_last_error;
+
+static pis_status fail(pis_status s, const std::string& msg)
+{
+    g_last_error = msg;
+    return s;
+}
+
+#define PIS_HIP(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(e_ == hipErrorOutOfMemory ? PIS_ERR_OUT_OF_MEMORY : PIS_ERR_HIP,     \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+
+#define PIS_TRY(expr)                    \
+    do {                                 \
+        pis_status s_ = (expr);          \
+        if (s_ != PIS_OK) return s_;     \
+    } while (0)
+
+// ---------------------------------------------------------------------------------
+// planning (host only)
+
+namespace {
+
+// Largest double r2 with sqrt_rn(r2) <= rc, so that `norm() > rc` (the reference's
+// rejection, lennard_jones.rs:224) is exactly `r2 > r2max` for the same r2 bits.
+double r2max_for(double rc)
+{
+    if (!(rc > 0.0) || !std::isfinite(rc)) return -1.0;
+    double r2 = rc * rc;
+    while (std::sqrt(r2) > rc) r2 = std::nextafter(r2, 0.0);
+    for (;;) {
+        const double nx = std::nextafter(r2, INFINITY);
+        if (std::sqrt(nx) <= rc)
+            r2 = nx;
+        else
+            break;
+    }
+    return r2;
+}
+
+struct Plan {
+    int64_t n = 0;
+    int ntypes = 0;
+    int nc[3] = {1, 1, 1}, tile[3] = {1, 1, 1}, nt[3] = {1, 1, 1};
+    int ncell = 1, ntile = 1, mode = MODE_WHOLE, maxnb = 8, scap = 1024, npad = 64;
+    double L[3] = {0, 0, 0}, Linv[3] = {0, 0, 0}, cellw[3] = {0, 0, 0};
+    double rcmax = 0, rcl = 0, skin = 0, density = 0, volume = 0, kb = 1;
+    bool mixed = false;
+    std::vector<PairParam> pp;
+    std::vector<double> mass;
+    std::vector<uint8_t> type0;
+};
+
+int scap_for(double need)
+{
+    const int opts[4] = {1024, 1536, 2048, 4096};
+    for (int k = 0; k < 4; ++k)
+        if (need <= opts[k]) return opts[k];
+    return -1;
+}
+
+// Global tile override used by the autotuner (0 = heuristic).
+int g_tile_override[3] = {0, 0, 0};
+
+// LDS bytes of a tile kernel instance with staging capacity `scap`.
+double tile_lds(int scap) { return (scap + 1) * 25.0 + 2300.0; }
+
+// Tile choice: maximise lane use of the 256-thread workgroup times an occupancy factor
+// (workgroups per CU the LDS allows, capped at the 3 the register file allows), with a
+// small penalty for halo re-staging.  Staging capacity gets a margin for density
+// fluctuations; an overflow at run time is detected and reported, never silent.
+void choose_tiles(Plan& p)
+{
+    const double cellvol = p.cellw[0] * p.cellw[1] * p.cellw[2];
+    double best = -1e300;
+    int bt[3] = {1, 1, 1};
+    const int lim[3] = {std::max(1, p.nc[0] - 2), std::max(1, p.nc[1] - 2), std::max(1, p.nc[2] - 2)};
+    for (int tz = 1; tz <= std::min(lim[2], 8); ++tz)
+        for (int ty = 1; ty <= std::min(lim[1], 8); ++ty)
+            for (int tx = 1; tx <= std::min(lim[0], 8); ++tx) {
+                if ((tx + 2) * (ty + 2) * (tz + 2) > kMaxHalo) continue;
+                const double own = p.density * cellvol * tx * ty * tz;
+                const double stage = p.density * cellvol * (tx + 2) * (ty + 2) * (tz + 2);
+                const int sc = scap_for(stage * 1.1 + 48);
+                if (sc < 0) continue;
+                const double passes = std::ceil((own * 1.06 + 4) / kTileThreads);
+                const double eff = own / (passes * kTileThreads);
+                const double wg = std::min(3.0, std::floor(163840.0 / tile_lds(sc)));
+                const double score = eff * (0.5 + 0.5 * wg / 3.0) - 0.004 * (stage / std::max(own, 1.0)) -
+                                     0.01 * (std::max(tx, std::max(ty, tz)) - std::min(tx, std::min(ty, tz)));
+                if (score > best) {
+                    best = score;
+                    bt[0] = tx;
+                    bt[1] = ty;
+                    bt[2] = tz;
+                }
+            }
+    if (g_tile_override[0] > 0) {
+        int ov[3];
+        for (int d = 0; d < 3; ++d) ov[d] = std::max(1, std::min(g_tile_override[d], lim[d]));
+        if ((ov[0] + 2) * (ov[1] + 2) * (ov[2] + 2) <= kMaxHalo)
+            for (int d = 0; d < 3; ++d) bt[d] = ov[d];
+    }
+    for (int d = 0; d < 3; ++d) p.tile[d] = bt[d];
+}
+
+pis_status make_plan(const pis_system_desc* d, Plan* p)
+{
+    if (!d) return fail(PIS_ERR_INVALID_ARGUMENT, "null system descriptor");
+    if (d->n_atoms == 0) return fail(PIS_ERR_NO_ATOMS, "No atoms defined in input file");
+    if (d->n_atoms > (uint64_t)1 << 30) return fail(PIS_ERR_UNSUPPORTED, "more than 2^30 atoms");
+    if (d->flags != 0) return fail(PIS_ERR_INVALID_ARGUMENT, "pis_system_desc.flags must be 0");
+    if (d->n_types == 0 || d->n_types > (uint32_t)kMaxTypes)
+        return fail(PIS_ERR_INVALID_ATOM_TYPE, "n_types must be 1.." + std::to_string(kMaxTypes));
+    if (!d->type_ids || !d->masses) return fail(PIS_ERR_INVALID_ARGUMENT, "type_ids/masses missing");
+    if (d->n_pairs == 0 || !d->pairs)
+        return fail(PIS_ERR_MISSING_POTENTIAL,
+                    "Potential manager not initialized - missing pair_style or pair_coeff commands");
+    if (d->precision > 1) return fail(PIS_ERR_INVALID_ARGUMENT, "unknown precision");
+    if (!d->pbc[0] || !d->pbc[1] || !d->pbc[2])
+        return fail(PIS_ERR_UNSUPPORTED, "only fully periodic boxes are supported");
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            if (r != c && (d->h[c * 3 + r] != 0.0 || d->h_inv[c * 3 + r] != 0.0))
+                return fail(PIS_ERR_UNSUPPORTED, "triclinic boxes are not supported");
+    if (!(d->skin >= 0.0) || !std::isfinite(d->skin)) return fail(PIS_ERR_INVALID_ARGUMENT, "bad skin");
+    if (!(d->kb > 0.0)) return fail(PIS_ERR_INVALID_ARGUMENT, "kb must be > 0");
+    const int nt = (int)d->n_types;
+    p->n = (int64_t)d->n_atoms;
+    p->ntypes = nt;
+    p->skin = d->skin;
+    p->kb = d->kb;
+    p->mixed = d->precision == PIS_PRECISION_MIXED;
+    for (int k = 0; k < 3; ++k) {
+        p->L[k] = d->h[k * 3 + k];
+        p->Linv[k] = d->h_inv[k * 3 + k];
+        if (!(p->L[k] > 0.0) || !(p->Linv[k] > 0.0) || !std::isfinite(p->L[k]))
+            return fail(PIS_ERR_INVALID_ARGUMENT, "box lengths must be positive");
+    }
+    p->volume = std::fabs(p->L[0] * p->L[1] * p->L[2]);
+    p->mass.resize(nt);
+    for (int t = 0; t < nt; ++t) {
+        if (!(d->masses[t] > 0.0)) return fail(PIS_ERR_INVALID_ARGUMENT, "masses must be > 0");
+        p->mass[t] = d->masses[t];
+    }
+    p->type0.resize(p->n);
+    for (int64_t i = 0; i < p->n; ++i) {
+        const uint32_t t = d->type_ids[i];
+        if (t < 1 || t > (uint32_t)nt)
+            return fail(PIS_ERR_INVALID_ATOM_TYPE, "Atom type " + std::to_string(t) + " out of range");
+        p->type0[i] = (uint8_t)(t - 1);
+    }
+    // pair table: (min,max) keyed like the reference; absent pairs never interact
+    p->pp.assign((size_t)nt * nt, PairParam{});
+    for (auto& q : p->pp) {
+        q.r2_force = -1.0;
+        q.r2_list = -1.0;
+    }
+    p->rcmax = 0.0;
+    for (uint32_t k = 0; k < d->n_pairs; ++k) {
+        const pis_pair_coeff& c = d->pairs[k];
+        if (c.type_i < 1 || c.type_j < 1 || c.type_i > (uint32_t)nt || c.type_j > (uint32_t)nt)
+            return fail(PIS_ERR_INVALID_ATOM_TYPE, "pair_coeff type out of range");
+        if (!(c.rcut > 0.0) || !(c.sigma > 0.0) || !std::isfinite(c.epsilon))
+            return fail(PIS_ERR_INVALID_ARGUMENT, "pair_coeff needs sigma > 0, rcut > 0");
+        const int a = (int)std::min(c.type_i, c.type_j) - 1, b = (int)std::max(c.type_i, c.type_j) - 1;
+        PairParam q;
+        q.r2_force = r2max_for(c.rcut);
+        q.r2_list = r2max_for(c.rcut + d->skin);
+        q.sig2 = c.sigma * c.sigma;
+        q.c4 = 4.0 * c.epsilon;
+        q.c24 = 24.0 * c.epsilon;
+        q.c48 = 48.0 * c.epsilon;
+        q.ucut = 0.0;
+        if (d->shift) {  // lennard_jones.rs:44-52
+            const double ci2 = (c.sigma / c.rcut) * (c.sigma / c.rcut);
+            const double ca = ci2 * (ci2 * ci2);
+            const double cr = ca * ca;
+            q.ucut = 4.0 * c.epsilon * (cr - ca);
+        }
+        q.sig2f = (float)q.sig2;
+        q.c4f = (float)q.c4;
+        q.c24f = (float)q.c24;
+        q.c48f = (float)q.c48;
+        q.ucutf = (float)q.ucut;
+        q.pad = 0.0f;
+        p->pp[(size_t)a * nt + b] = q;
+        p->pp[(size_t)b * nt + a] = q;
+        p->rcmax = std::max(p->rcmax, c.rcut);
+    }
+    p->rcl = p->rcmax + d->skin;
+    p->density = (double)p->n / p->volume;
+    // Cell grid: edge strictly larger than rcut + skin (relative margin far above
+    // rounding) so every pair within the list cutoff is in adjacent cells.
+    bool thin = false;
+    for (int k = 0; k < 3; ++k) {
+        const double nf = std::floor(p->L[k] / (p->rcl * (1.0 + 1e-12)));
+        p->nc[k] = (int)std::max(1.0, std::min(nf, 1024.0));
+        p->cellw[k] = p->L[k] / p->nc[k];
+        if (p->nc[k] < 3) thin = true;
+    }
+    p->ncell = p->nc[0] * p->nc[1] * p->nc[2];
+    const double vsphere = 4.0 / 3.0 * M_PI * p->rcl * p->rcl * p->rcl;
+    if (thin) {
+        p->mode = MODE_WHOLE;
+        for (int k = 0; k < 3; ++k) {
+            p->tile[k] = p->nc[k];
+            p->nt[k] = 1;
+        }
+        p->scap = scap_for((double)p->n + 1);
+        if (p->scap < 0)
+            return fail(PIS_ERR_UNSUPPORTED,
+                        "box thinner than 3 cells of rcut+skin holds more than 4095 atoms");
+    } else {
+        choose_tiles(*p);
+        const double cellvol = p->cellw[0] * p->cellw[1] * p->cellw[2];
+        p->mode = MODE_TILE_REL;
+        for (int k = 0; k < 3; ++k) {
+            p->nt[k] = (p->nc[k] + p->tile[k] - 1) / p->tile[k];
+            // tile-relative staging needs half the tile + skin/2 + rcut below L/2
+            if (!(p->tile[k] * p->cellw[k] + p->skin + 2.0 * p->rcmax < p->L[k] * (1.0 - 1e-9)))
+                p->mode = MODE_RAW_MI;
+        }
+        const double stage = p->density * cellvol * (p->tile[0] + 2) * (p->tile[1] + 2) * (p->tile[2] + 2);
+        p->scap = scap_for(stage * 1.1 + 48);
+        if (p->scap < 0) return fail(PIS_ERR_UNSUPPORTED, "tile staging exceeds LDS capacity");
+    }
+    p->ntile = p->nt[0] * p->nt[1] * p->nt[2];
+    p->npad = (int)((p->n + 63) / 64 * 64);
+    int maxnb;
+    if (d->neighbor_capacity)
+        maxnb = (int)d->neighbor_capacity;
+    else
+        maxnb = (int)std::ceil(1.6 * p->density * vsphere) + 48;
+    maxnb = std::min(maxnb, 65528);
+    if (p->mode == MODE_WHOLE) maxnb = std::min(maxnb, (int)p->n + 8);
+    p->maxnb = std::max(8, (maxnb + 7) / 8 * 8);
+    return PIS_OK;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------
+// context
+
+struct SysHost {
+    Plan plan;
+    void* mem = nullptr;
+    size_t mem_bytes = 0;
+    ThermoDev* thermo = nullptr;
+    int thermo_cap = 0;
+    SysDev dev{};
+};
+
+struct pis_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<SysHost> sys;
+    SysDev* d_sys = nullptr;
+    WorkItem* d_atom_work = nullptr;
+    int n_atom_work = 0;
+    WorkItem* d_tile_work = nullptr;
+    int n_tile_work = 0;
+    int scap = 1024;
+    bool mixed = false;
+    bool multi = false;
+    void* d_xfer = nullptr;
+    size_t xfer_bytes = 0;
+    // statistics / timing
+    uint64_t steps = 0, force_launches = 0;
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    double force_ms = 0.0;
+    uint64_t timed_launches = 0;
+};
+
+static pis_status alloc_system(SysHost& h)
+{
+    const Plan& p = h.plan;
+    const size_t np = (size_t)p.npad, nc = (size_t)p.ncell;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += align256(bytes);
+        return o;
+    };
+    size_t o_st[2][11];
+    for (int s = 0; s < 2; ++s) {
+        for (int k = 0; k < 9; ++k) o_st[s][k] = take(np * 8);
+        o_st[s][9] = take(np);
+        o_st[s][10] = take(np * 4);
+    }
+    const size_t o_r = take(np * 8 * 3);
+    const size_t o_ctl = take(64 * 4);
+    const size_t o_cc = take(nc * 4), o_cf = take(nc * 4), o_cs = take((nc + 1) * 4);
+    const size_t o_n2s = take(nc * 4), o_tc = take(((size_t)p.ntile + 1) * 4);
+    const size_t o_ac = take(np * 4), o_ss = take(np * 4), o_sk = take(np * 4);
+    const size_t o_nbr = take((size_t)(p.maxnb / 8) * np * 16);
+    const size_t o_nnb = take(np * 2);
+    const size_t o_part = take((size_t)p.ntile * 4 * 8);
+    const size_t o_pp = take(p.pp.size() * sizeof(PairParam));
+    const size_t o_m = take(p.mass.size() * 8);
+    void* mem = nullptr;
+    PIS_HIP(hipMalloc(&mem, off));
+    h.mem = mem;
+    h.mem_bytes = off;
+    char* b = (char*)mem;
+    SysDev& d = h.dev;
+    for (int s = 0; s < 2; ++s) {
+        double** f[9] = {&d.st[s].x, &d.st[s].y, &d.st[s].z, &d.st[s].vx, &d.st[s].vy,
+                         &d.st[s].vz, &d.st[s].fx, &d.st[s].fy, &d.st[s].fz};
+        for (int k = 0; k < 9; ++k) *f[k] = (double*)(b + o_st[s][k]);
+        d.st[s].type = (uint8_t*)(b + o_st[s][9]);
+        d.st[s].orig = (uint32_t*)(b + o_st[s][10]);
+    }
+    d.rx = (double*)(b + o_r);
+    d.ry = d.rx + np;
+    d.rz = d.ry + np;
+    d.ctl = (int32_t*)(b + o_ctl);
+    d.cell_count = (int32_t*)(b + o_cc);
+    d.cell_fill = (int32_t*)(b + o_cf);
+    d.cell_start = (int32_t*)(b + o_cs);
+    d.nat2sorted = (int32_t*)(b + o_n2s);
+    d.tile_cell0 = (int32_t*)(b + o_tc);
+    d.atom_cell = (int32_t*)(b + o_ac);
+    d.scat_src = (int32_t*)(b + o_ss);
+    d.scat_key = (uint32_t*)(b + o_sk);
+    d.nbr = (uint4*)(b + o_nbr);
+    d.nnb = (uint16_t*)(b + o_nnb);
+    d.partial = (double*)(b + o_part);
+    d.pp = (const PairParam*)(b + o_pp);
+    d.mass = (const double*)(b + o_m);
+    d.thermo = nullptr;
+    d.thermo_cap = 0;
+    for (int k = 0; k < 3; ++k) {
+        d.L[k] = p.L[k];
+        d.Linv[k] = p.Linv[k];
+        d.cellw[k] = p.cellw[k];
+        d.nc[k] = p.nc[k];
+        d.tile[k] = p.tile[k];
+        d.nt[k] = p.nt[k];
+    }
+    d.half_skin2 = 0.25 * p.skin * p.skin;
+    d.dt = 0.0;
+    d.dt2 = 0.0;
+    d.kb = p.kb;
+    d.volume = p.volume;
+    d.n = (int32_t)p.n;
+    d.npad = p.npad;
+    d.ntypes = p.ntypes;
+    d.maxnb = p.maxnb;
+    d.ncell = p.ncell;
+    d.ntile = p.ntile;
+    d.mode = p.mode;
+    // zero everything once (outside any captured sequence), then tables
+    PIS_HIP(hipMemset(mem, 0, off));
+    // natural -> sorted (tile-major) cell ids and the first sorted cell of every tile
+    std::vector<int32_t> n2s((size_t)p.ncell), tc((size_t)p.ntile + 1);
+    int32_t next = 0;
+    for (int tz = 0; tz < p.nt[2]; ++tz)
+        for (int ty = 0; ty < p.nt[1]; ++ty)
+            for (int tx = 0; tx < p.nt[0]; ++tx) {
+                const int t = (tz * p.nt[1] + ty) * p.nt[0] + tx;
+                tc[t] = next;
+                const int x0 = tx * p.tile[0], y0 = ty * p.tile[1], z0 = tz * p.tile[2];
+                const int sx = std::min(p.tile[0], p.nc[0] - x0), sy = std::min(p.tile[1], p.nc[1] - y0),
+                          sz = std::min(p.tile[2], p.nc[2] - z0);
+                for (int lz = 0; lz < sz; ++lz)
+                    for (int ly = 0; ly < sy; ++ly)
+                        for (int lx = 0; lx < sx; ++lx)
+                            n2s[((size_t)(z0 + lz) * p.nc[1] + (y0 + ly)) * p.nc[0] + (x0 + lx)] = next++;
+            }
+    tc[p.ntile] = next;
+    PIS_HIP(hipMemcpy(d.nat2sorted, n2s.data(), n2s.size() * 4, hipMemcpyHostToDevice));
+    PIS_HIP(hipMemcpy(d.tile_cell0, tc.data(), tc.size() * 4, hipMemcpyHostToDevice));
+    PIS_HIP(hipMemcpy((void*)d.pp, p.pp.data(), p.pp.size() * sizeof(PairParam), hipMemcpyHostToDevice));
+    PIS_HIP(hipMemcpy((void*)d.mass, p.mass.data(), p.mass.size() * 8, hipMemcpyHostToDevice));
+    return PIS_OK;
+}
+
+static pis_status upload_sys(pis_ctx* c)
+{
+    std::vector<SysDev> v(c->sys.size());
+    for (size_t k = 0; k < c->sys.size(); ++k) v[k] = c->sys[k].dev;
+    PIS_HIP(hipMemcpyAsync(c->d_sys, v.data(), v.size() * sizeof(SysDev), hipMemcpyHostToDevice, c->stream));
+    PIS_HIP(hipStreamSynchronize(c->stream));
+    return PIS_OK;
+}
+
+static void free_ctx(pis_ctx* c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& s : c->sys) {
+        if (s.mem) (void)hipFree(s.mem);
+        if (s.thermo) (void)hipFree(s.thermo);
+    }
+    if (c->d_sys) (void)hipFree(c->d_sys);
+    if (c->d_atom_work) (void)hipFree(c->d_atom_work);
+    if (c->d_tile_work) (void)hipFree(c->d_tile_work);
+    if (c->d_xfer) (void)hipFree(c->d_xfer);
+    for (auto e : c->ev) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+static pis_status create_ctx(const pis_system_desc* descs, uint32_t nsys, pis_ctx** out)
+{
+    if (!out) return fail(PIS_ERR_INVALID_ARGUMENT, "null output pointer");
+    *out = nullptr;
+    if (!descs || nsys == 0) return fail(PIS_ERR_INVALID_ARGUMENT, "no systems");
+    std::vector<Plan> plans(nsys);
+    for (uint32_t k = 0; k < nsys; ++k) {
+        pis_status s = make_plan(&descs[k], &plans[k]);
+        if (s != PIS_OK) return fail(s, "system " + std::to_string(k) + ": " + g_last_error);
+        if (plans[k].mixed != plans[0].mixed)
+            return fail(PIS_ERR_INVALID_ARGUMENT, "all systems of a batch need the same precision");
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(PIS_ERR_NO_DEVICE, "no HIP device available");
+    pis_ctx* c = new pis_ctx();
+    c->device = 0;
+    hipError_t e = hipSetDevice(c->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        free_ctx(c);
+        return fail(PIS_ERR_HIP, std::string("stream: ") + hipGetErrorString(e));
+    }
+    c->sys.resize(nsys);
+    c->mixed = plans[0].mixed;
+    std::vector<WorkItem> aw, tw;
+    for (uint32_t k = 0; k < nsys; ++k) {
+        c->sys[k].plan = std::move(plans[k]);
+        pis_status s = alloc_system(c->sys[k]);
+        if (s != PIS_OK) {
+            free_ctx(c);
+            return s;
+        }
+        const Plan& p = c->sys[k].plan;
+        c->scap = std::max(c->scap, p.scap);
+        c->multi |= p.ntypes > 1;
+        for (int64_t a = 0; a < p.n; a += kAtomThreads) aw.push_back(WorkItem{(int32_t)k, (int32_t)a});
+        for (int t = 0; t < p.ntile; ++t) tw.push_back(WorkItem{(int32_t)k, t});
+    }
+    c->n_atom_work = (int)aw.size();
+    c->n_tile_work = (int)tw.size();
+    e = hipMalloc(&c->d_sys, sizeof(SysDev) * nsys);
+    if (e == hipSuccess) e = hipMalloc(&c->d_atom_work, sizeof(WorkItem) * aw.size());
+    if (e == hipSuccess) e = hipMalloc(&c->d_tile_work, sizeof(WorkItem) * tw.size());
+    if (e == hipSuccess) e = hipMemcpy(c->d_atom_work, aw.data(), sizeof(WorkItem) * aw.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->d_tile_work, tw.data(), sizeof(WorkItem) * tw.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        free_ctx(c);
+        return fail(e == hipErrorOutOfMemory ? PIS_ERR_OUT_OF_MEMORY : PIS_ERR_HIP,
+                    std::string("context allocation: ") + hipGetErrorString(e));
+    }
+    pis_status s = upload_sys(c);
+    if (s != PIS_OK) {
+        free_ctx(c);
+        return s;
+    }
+    *out = c;
+    return PIS_OK;
+}
+
+static pis_status ensure_xfer(pis_ctx* c, size_t bytes)
+{
+    if (c->xfer_bytes >= bytes) return PIS_OK;
+    if (c->d_xfer) PIS_HIP(hipFree(c->d_xfer));
+    c->d_xfer = nullptr;
+    c->xfer_bytes = 0;
+    PIS_HIP(hipMalloc(&c->d_xfer, bytes));
+    c->xfer_bytes = bytes;
+    return PIS_OK;
+}
+
+static pis_status read_ctl(pis_ctx* c, uint32_t k, int32_t* ctl)
+{
+    PIS_HIP(hipMemcpyAsync(ctl, c->sys[k].dev.ctl, CTL_COUNT * 4, hipMemcpyDeviceToHost, c->stream));
+    PIS_HIP(hipStreamSynchronize(c->stream));
+    return PIS_OK;
+}
+
+static pis_status write_ctl(pis_ctx* c, uint32_t k, const int32_t* ctl)
+{
+    PIS_HIP(hipMemcpyAsync(c->sys[k].dev.ctl, ctl, CTL_COUNT * 4, hipMemcpyHostToDevice, c->stream));
+    PIS_HIP(hipStreamSynchronize(c->stream));
+    return PIS_OK;
+}
+
+static pis_status run_rebuild(pis_ctx* c)
+{
+    launch_rebuild(c->stream, c->d_sys, (int)c->sys.size(), c->d_atom_work, c->n_atom_work,
+                   c->d_tile_work, c->n_tile_work, c->scap, c->multi);
+    launch_flip(c->stream, c->d_sys, (int)c->sys.size());
+    PIS_HIP(hipGetLastError());
+    return PIS_OK;
+}
+
+static pis_status check_flags(pis_ctx* c)
+{
+    for (uint32_t k = 0; k < c->sys.size(); ++k) {
+        int32_t ctl[CTL_COUNT];
+        PIS_TRY(read_ctl(c, k, ctl));
+        if (ctl[CTL_OVF_STAGE])
+            return fail(PIS_ERR_CAPACITY, "system " + std::to_string(k) + ": LDS staging capacity exceeded");
+        if (ctl[CTL_OVF_NB])
+            return fail(PIS_ERR_CAPACITY, "system " + std::to_string(k) + ": neighbour capacity " +
+                                              std::to_string(c->sys[k].plan.maxnb) + " exceeded (" +
+                                              std::to_string(ctl[CTL_MAXNB]) + " needed)");
+        if (ctl[CTL_NAN])
+            return fail(PIS_ERR_NAN_ENERGY, "system " + std::to_string(k) + ": non-finite energy or position");
+    }
+    return PIS_OK;
+}
+
+static pis_status sys_set_state(pis_ctx* c, uint32_t k, const double* pos, const double* vel,
+                                const double* force)
+{
+    if (!c) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
+    if (k >= c->sys.size()) return fail(PIS_ERR_INVALID_ARGUMENT, "system index out of range");
+    if (!pos) return fail(PIS_ERR_INVALID_ARGUMENT, "positions are required");
+    SysHost& h = c->sys[k];
+    const size_t n = (size_t)h.plan.n;
+    for (size_t i = 0; i < 3 * n; ++i)
+        if (!std::isfinite(pos[i])) return fail(PIS_ERR_INVALID_ARGUMENT, "non-finite position");
+    PIS_TRY(ensure_xfer(c, 3 * 3 * n * 8 + n + 256));
+    double* dp = (double*)c->d_xfer;
+    double* dv = dp + 3 * n;
+    double* df = dv + 3 * n;
+    uint8_t* dt = (uint8_t*)(df + 3 * n);
+    PIS_HIP(hipMemcpyAsync(dp, pos, 3 * n * 8, hipMemcpyHostToDevice, c->stream));
+    if (vel) PIS_HIP(hipMemcpyAsync(dv, vel, 3 * n * 8, hipMemcpyHostToDevice, c->stream));
+    if (force) PIS_HIP(hipMemcpyAsync(df, force, 3 * n * 8, hipMemcpyHostToDevice, c->stream));
+    PIS_HIP(hipMemcpyAsync(dt, h.plan.type0.data(), n, hipMemcpyHostToDevice, c->stream));
+    int32_t ctl[CTL_COUNT] = {0};
+    ctl[CTL_CUR] = 0;
+    ctl[CTL_REBUILD] = 1;
+    PIS_TRY(write_ctl(c, k, ctl));
+    launch_load_aos(c->stream, c->d_sys, (int)k, (int)n, dp, vel ? dv : nullptr, force ? df : nullptr, dt);
+    PIS_TRY(run_rebuild(c));
+    PIS_HIP(hipStreamSynchronize(c->stream));
+    return check_flags(c);
+}
+
+static pis_status sys_get_state(pis_ctx* c, uint32_t k, double* pos, double* vel, double* force)
+{
+    if (!c) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
+    if (k >= c->sys.size()) return fail(PIS_ERR_INVALID_ARGUMENT, "system index out of range");
+    const size_t n = (size_t)c->sys[k].plan.n;
+    PIS_TRY(ensure_xfer(c, 3 * 3 * n * 8 + n + 256));
+    double* dp = (double*)c->d_xfer;
+    double* dv = dp + 3 * n;
+    double* df = dv + 3 * n;
+    launch_store_aos(c->stream, c->d_sys, (int)k, (int)n, dp, dv, df);
+    PIS_HIP(hipGetLastError());
+    if (pos) PIS_HIP(hipMemcpyAsync(pos, dp, 3 * n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (vel) PIS_HIP(hipMemcpyAsync(vel, dv, 3 * n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (force) PIS_HIP(hipMemcpyAsync(force, df, 3 * n * 8, hipMemcpyDeviceToHost, c->stream));
+    PIS_HIP(hipStreamSynchronize(c->stream));
+    return PIS_OK;
+}
+
+static pis_status ensure_thermo(pis_ctx* c, int cap)
+{
+    bool changed = false;
+    for (auto& h : c->sys) {
+        if (h.thermo_cap >= cap) continue;
+        if (h.thermo) PIS_HIP(hipFree(h.thermo));
+        h.thermo = nullptr;
+        PIS_HIP(hipMalloc(&h.thermo, sizeof(ThermoDev) * (size_t)cap));
+        h.thermo_cap = cap;
+        h.dev.thermo = h.thermo;
+        h.dev.thermo_cap = cap;
+        changed = true;
+    }
+    if (changed) PIS_TRY(upload_sys(c));
+    return PIS_OK;
+}
+
+static pis_status launch_force_timed(pis_ctx* c, int integrate)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+        PIS_HIP(hipEventCreate(&e0));
+        PIS_HIP(hipEventCreate(&e1));
+        PIS_HIP(hipEventRecord(e0, c->stream));
+    }
+    launch_force(c->stream, c->d_sys, c->d_tile_work, c->n_tile_work, c->scap, c->multi, c->mixed, integrate);
+    if (c->timing) {
+        PIS_HIP(hipEventRecord(e1, c->stream));
+        c->ev.push_back(e0);
+        c->ev.push_back(e1);
+    }
+    c->force_launches++;
+    return PIS_OK;
+}
+
+static pis_status collect_timing(pis_ctx* c)
+{
+    for (size_t k = 0; k + 1 < c->ev.size(); k += 2) {
+        float ms = 0.0f;
+        PIS_HIP(hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]));
+        c->force_ms += ms;
+        c->timed_launches++;
+    }
+    for (auto e : c->ev) (void)hipEventDestroy(e);
+    c->ev.clear();
+    return PIS_OK;
+}
+
+static pis_status read_thermo(pis_ctx* c, uint64_t nrec, pis_thermo* out)
+{
+    const size_t ns = c->sys.size();
+    std::vector<ThermoDev> buf(nrec);
+    for (size_t k = 0; k < ns; ++k) {
+        PIS_HIP(hipMemcpyAsync(buf.data(), c->sys[k].thermo, sizeof(ThermoDev) * nrec,
+                               hipMemcpyDeviceToHost, c->stream));
+        PIS_HIP(hipStreamSynchronize(c->stream));
+        for (uint64_t t = 0; t < nrec; ++t) {
+            pis_thermo& o = out[t * ns + k];
+            o.pe = buf[t].pe;
+            o.ke = buf[t].ke;
+            o.etot = buf[t].etot;
+            o.temperature = buf[t].temperature;
+            o.pressure = buf[t].pressure;
+            o.virial_abs = buf[t].virial_abs;
+            o.virial_pair = buf[t].virial_pair;
+        }
+    }
+    return PIS_OK;
+}
+
+static pis_status batch_compute(pis_ctx* c, pis_thermo* out /* [nsys] or null */)
+{
+    if (!c) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
+    PIS_TRY(ensure_thermo(c, 1));
+    launch_set_ctl(c->stream, c->d_sys, (int)c->sys.size(), CTL_STEP, 0);
+    PIS_TRY(launch_force_timed(c, 0));
+    launch_reduce(c->stream, c->d_sys, (int)c->sys.size(), 1, 1);
+    PIS_HIP(hipGetLastError());
+    PIS_HIP(hipStreamSynchronize(c->stream));
+    PIS_TRY(collect_timing(c));
+    PIS_TRY(check_flags(c));
+    std::vector<pis_thermo> t(c->sys.size());
+    PIS_TRY(read_thermo(c, 1, t.data()));
+    if (out)
+        for (size_t k = 0; k < t.size(); ++k) out[k] = t[k];
+    return PIS_OK;
+}
+
+static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_thermo* out)
+{
+    if (!c) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
+    if (!dts) return fail(PIS_ERR_INVALID_ARGUMENT, "null timestep array");
+    if (nsteps == 0) return PIS_OK;
+    if (nsteps > (uint64_t)1 << 30) return fail(PIS_ERR_INVALID_ARGUMENT, "too many steps");
+    for (size_t k = 0; k < c->sys.size(); ++k) {
+        if (!(dts[k] > 0.0) || !std::isfinite(dts[k])) return fail(PIS_ERR_INVALID_ARGUMENT, "dt must be > 0");
+        c->sys[k].dev.dt = dts[k];
+        c->sys[k].dev.dt2 = dts[k] * dts[k];  // dt.powi(2)
+    }
+    const int cap = (int)std::min<uint64_t>(nsteps, (uint64_t)1 << 30);
+    PIS_TRY(ensure_thermo(c, out ? cap : 1));
+    PIS_TRY(upload_sys(c));
+    const int ns = (int)c->sys.size();
+    launch_set_ctl(c->stream, c->d_sys, ns, CTL_STEP, 0);
+    for (uint64_t t = 0; t < nsteps; ++t) {
+        launch_integrate_pos(c->stream, c->d_sys, c->d_atom_work, c->n_atom_work);
+        launch_rebuild(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work, c->d_tile_work,
+                       c->n_tile_work, c->scap, c->multi);
+        PIS_TRY(launch_force_timed(c, 1));
+        launch_reduce(c->stream, c->d_sys, ns, 1, out ? 1 : 0);
+        if ((t & 255) == 0) PIS_HIP(hipGetLastError());
+    }
+    PIS_HIP(hipGetLastError());
+    PIS_HIP(hipStreamSynchronize(c->stream));
+    c->steps += nsteps;
+    PIS_TRY(collect_timing(c));
+    PIS_TRY(check_flags(c));
+    if (out) PIS_TRY(read_thermo(c, nsteps, out));
+    return PIS_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+int pis_abi_version(void) { return PIS_ABI_VERSION; }
+
+const char* pis_last_error(void) { return g_last_error.c_str(); }
+
+const char* pis_status_string(pis_status s)
+{
+    switch (s) {
+    case PIS_OK: return "ok";
+    case PIS_ERR_NO_ATOMS: return "no atoms defined";
+    case PIS_ERR_MISSING_POTENTIAL: return "missing potential";
+    case PIS_ERR_NAN_ENERGY: return "NaN energy";
+    case PIS_ERR_INVALID_ATOM_TYPE: return "invalid atom type";
+    case PIS_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case PIS_ERR_UNSUPPORTED: return "unsupported";
+    case PIS_ERR_HIP: return "HIP error";
+    case PIS_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case PIS_ERR_NO_DEVICE: return "no device";
+    case PIS_ERR_CAPACITY: return "capacity exceeded";
+    case PIS_ERR_NOT_FOUND: return "not found";
+    }
+    return "unknown status";
+}
+
+pis_status pis_device_count(int32_t* count)
+{
+    if (!count) return fail(PIS_ERR_INVALID_ARGUMENT, "null count");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return PIS_OK;
+}
+
+pis_status pis_plan_system(const pis_system_desc* desc, pis_plan_info* out)
+{
+    if (!out) return fail(PIS_ERR_INVALID_ARGUMENT, "null output");
+    Plan p;
+    PIS_TRY(make_plan(desc, &p));
+    for (int k = 0; k < 3; ++k) {
+        out->ncell[k] = p.nc[k];
+        out->tile[k] = p.tile[k];
+        out->ntile[k] = p.nt[k];
+    }
+    out->mode = p.mode;
+    out->neighbor_capacity = p.maxnb;
+    out->stage_capacity = p.scap;
+    out->r2_list_max = r2max_for(p.rcl);
+    out->density = p.density;
+    return PIS_OK;
+}
+
+pis_status pis_set_tile_override(const int32_t* tile)
+{
+    for (int d = 0; d < 3; ++d) g_tile_override[d] = tile ? std::max(0, tile[d]) : 0;
+    return PIS_OK;
+}
+
+pis_status pis_ctx_create(const pis_system_desc* desc, pis_ctx** out) { return create_ctx(desc, 1, out); }
+
+pis_status pis_batch_create(const pis_system_desc* descs, uint32_t n_systems, pis_ctx** out)
+{
+    return create_ctx(descs, n_systems, out);
+}
+
+uint32_t pis_batch_size(const pis_ctx* ctx) { return ctx ? (uint32_t)ctx->sys.size() : 0u; }
+
+void pis_ctx_destroy(pis_ctx* ctx) { free_ctx(ctx); }
+
+pis_status pis_set_state(pis_ctx* ctx, const double* pos, const double* vel, const double* force)
+{
+    return sys_set_state(ctx, 0, pos, vel, force);
+}
+
+pis_status pis_get_state(pis_ctx* ctx, double* pos, double* vel, double* force)
+{
+    return sys_get_state(ctx, 0, pos, vel, force);
+}
+
+pis_status pis_sys_set_state(pis_ctx* ctx, uint32_t sys, const double* pos, const double* vel,
+                             const double* force)
+{
+    return sys_set_state(ctx, sys, pos, vel, force);
+}
+
+pis_status pis_sys_get_state(pis_ctx* ctx, uint32_t sys, double* pos, double* vel, double* force)
+{
+    return sys_get_state(ctx, sys, pos, vel, force);
+}
+
+pis_status pis_compute_potential(pis_ctx* ctx, double* forces_inout, double* pe)
+{
+    if (!ctx) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
+    if (ctx->sys.size() != 1) return fail(PIS_ERR_INVALID_ARGUMENT, "use pis_batch_compute_potential");
+    pis_thermo t;
+    PIS_TRY(batch_compute(ctx, &t));
+    if (pe) *pe = t.pe;
+    if (forces_inout) {
+        const size_t n = (size_t)ctx->sys[0].plan.n;
+        std::vector<double> f(3 * n);
+        PIS_TRY(sys_get_state(ctx, 0, nullptr, nullptr, f.data()));
+        for (size_t i = 0; i < 3 * n; ++i) forces_inout[i] += f[i];
+    }
+    return PIS_OK;
+}
+
+pis_status pis_batch_compute_potential(pis_ctx* ctx, double* pe_out)
+{
+    if (!ctx) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
+    std::vector<pis_thermo> t(ctx->sys.size());
+    PIS_TRY(batch_compute(ctx, t.data()));
+    if (pe_out)
+        for (size_t k = 0; k < t.size(); ++k) pe_out[k] = t[k].pe;
+    return PIS_OK;
+}
+
+pis_status pis_verlet_step_nve(pis_ctx* ctx, double dt, uint64_t nsteps, pis_thermo* per_step_out)
+{
+    if (!ctx) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
+    if (ctx->sys.size() != 1) return fail(PIS_ERR_INVALID_ARGUMENT, "use pis_batch_verlet_step_nve");
+    return batch_run(ctx, &dt, nsteps, per_step_out);
+}
+
+pis_status pis_batch_verlet_step_nve(pis_ctx* ctx, const double* dts, uint64_t nsteps, pis_thermo* thermo_out)
+{
+    return batch_run(ctx, dts, nsteps, thermo_out);
+}
+
+pis_status pis_get_neighbor_list(pis_ctx* ctx, uint64_t* offsets, uint32_t* idx, uint64_t cap, uint64_t* total)
+{
+    if (!ctx) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
+    if (ctx->sys.size() != 1) return fail(PIS_ERR_INVALID_ARGUMENT, "single-system contexts only");
+    if (!offsets || !total) return fail(PIS_ERR_INVALID_ARGUMENT, "null output");
+    SysHost& h = ctx->sys[0];
+    int32_t ctl[CTL_COUNT];
+    PIS_TRY(read_ctl(ctx, 0, ctl));
+    ctl[CTL_REBUILD] = 1;
+    PIS_TRY(write_ctl(ctx, 0, ctl));
+    PIS_TRY(run_rebuild(ctx));
+    PIS_HIP(hipStreamSynchronize(ctx->stream));
+    PIS_TRY(check_flags(ctx));
+    const size_t n = (size_t)h.plan.n, mnb = (size_t)h.plan.maxnb;
+    uint32_t* d_out = nullptr;
+    PIS_HIP(hipMalloc(&d_out, (n * mnb + n) * 4));
+    launch_list_to_orig(ctx->stream, ctx->d_sys, ctx->d_tile_work, ctx->n_tile_work, ctx->scap,
+                        ctx->multi, d_out, d_out + n * mnb);
+    std::vector<uint32_t> buf(n * mnb + n);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(buf.data(), d_out, buf.size() * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return fail(PIS_ERR_HIP, std::string("list download: ") + hipGetErrorString(e));
+    const uint32_t* cnt = buf.data() + n * mnb;
+    offsets[0] = 0;
+    for (size_t i = 0; i < n; ++i) offsets[i + 1] = offsets[i] + cnt[i];
+    *total = offsets[n];
+    if (idx)
+        for (size_t i = 0; i < n; ++i)
+            for (uint32_t k = 0; k < cnt[i]; ++k) {
+                const uint64_t o = offsets[i] + k;
+                if (o < cap) idx[o] = buf[i * mnb + k];
+            }
+    return PIS_OK;
+}
+
+pis_status pis_get_stats(pis_ctx* ctx, pis_stats* out)
+{
+    if (!ctx || !out) return fail(PIS_ERR_INVALID_ARGUMENT, "null argument");
+    int32_t ctl[CTL_COUNT];
+    uint64_t rebuilds = 0;
+    int32_t mx = 0;
+    for (uint32_t k = 0; k < ctx->sys.size(); ++k) {
+        PIS_TRY(read_ctl(ctx, k, ctl));
+        rebuilds += (uint64_t)ctl[CTL_NREBUILD];
+        mx = std::max(mx, ctl[CTL_MAXNB]);
+    }
+    out->steps = ctx->steps;
+    out->rebuilds = rebuilds;
+    out->force_launches = ctx->force_launches;
+    out->force_kernel_ms = ctx->force_ms;
+    out->timed_force_launches = ctx->timed_launches;
+    out->neighbor_capacity = ctx->sys[0].plan.maxnb;
+    out->max_neighbors = mx;
+    return PIS_OK;
+}
+
+pis_status pis_set_kernel_timing(pis_ctx* ctx, int32_t enable)
+{
+    if (!ctx) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
+    ctx->timing = enable != 0;
+    ctx->force_ms = 0.0;
+    ctx->timed_launches = 0;
+    return PIS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,861 @@
+// kernels.hip -- gfx950 kernels of the LJ hot path.
+//
+// Compiled with -ffp-contract=off: every a*b+c that must round twice (the reference's
+// minimum image, wrap, binning, list predicate and integrator, which Rust never
+// contracts) does; fused multiply-adds appear only where written as fma().
+//
+// Per NVE step (one launch each, all systems of a batch together):
+//   k_integrate_pos   x += v dt + (F/m)/2 dt^2, wrap, skin check      (potential.rs:15-22)
+//   k_bin             cell of every atom, integer histogram           (neighbour_list.rs:9-42)
+//   k_scan            exclusive scan of the histogram per system
+//   k_scatter         atoms into cell slots (atomic slot, then...)
+//   k_rank_permute    ...deterministic rank by original index inside the cell, and the
+//                     permutation of all per-atom state into the other buffer set
+//   k_build_list      full neighbour list at rcut+skin, reference predicate bit-for-bit
+//                     (lennard_jones.rs:344-416)
+//   k_force           LDS-staged pair force + fused second half of the Verlet update
+//                     (lennard_jones.rs:32-55, 419-455; potential.rs:26-30)
+//   k_reduce          fixed-order per-system PE/KE/virial sums      (properties.rs:17-65)
+// The five rebuild kernels return at once unless the system's rebuild flag is set.
+#include <math.h>
+
+#include "kernels.h"
+
+namespace pis {
+
+// ---------------------------------------------------------------------------------
+// helpers
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range
+// of tiles so neighbouring tiles (which share halo atoms) hit the same L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblk)
+{
+    const int X = 8;
+    const int per = nblk / X, rem = nblk % X;
+    const int xcd = bid % X, loc = bid / X;
+    return xcd < rem ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
+}
+
+// Minimum image of one orthorhombic component, any tie rule (for staging and forces,
+// where a tie means |d| = L/2 > cutoff and the pair is rejected either way).
+__device__ __forceinline__ double mi_fast(double d, double L, double Linv)
+{
+    const double k = rint(d * Linv);
+    return fma(-k, L, d);
+}
+
+// apply_boundary_conditions_dis (simulation_box.rs:17-27) for an orthorhombic box:
+// s = h^-1 d, s -= round(s) (half away from zero), d = h s.  The off-diagonal terms of the
+// reference's gemv are signed zeros and do not change a nonzero result.
+__device__ __forceinline__ double mi_ref(double d, double L, double Linv)
+{
+    double s = d * Linv;
+    s = s - round(s);
+    return L * s;
+}
+
+// Exclusive scan over the block (blockDim.x a multiple of 64, <= 1024).
+__device__ int block_excl_scan(int v, int* tmp, int* total)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) tmp[wid] = x;
+    __syncthreads();
+    int add = 0, tot = 0;
+    for (int k = 0; k < nw; ++k) {
+        const int t = tmp[k];
+        add += (k < wid) ? t : 0;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return add + x - v;
+}
+
+// Deterministic sum of 4 doubles over a 256-thread block; result valid in thread 0.
+__device__ void block_sum4(double v[4], double* tmp /* [4 waves * 4] */)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double x = v[k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        v[k] = x;
+    }
+    if (lane == 0)
+        for (int k = 0; k < 4; ++k) tmp[wid * 4 + k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) {
+            double s = 0.0;
+            for (int w = 0; w < nw; ++w) s += tmp[w * 4 + k];
+            v[k] = s;
+        }
+}
+
+struct TileGeom {
+    int x0, y0, z0, sx, sy, sz, ne, c0, c1;
+};
+
+__device__ __forceinline__ TileGeom tile_geom(const SysDev& s, int t)
+{
+    TileGeom g;
+    const int tx = t % s.nt[0], ty = (t / s.nt[0]) % s.nt[1], tz = t / (s.nt[0] * s.nt[1]);
+    g.x0 = tx * s.tile[0];
+    g.y0 = ty * s.tile[1];
+    g.z0 = tz * s.tile[2];
+    g.sx = min(s.tile[0], s.nc[0] - g.x0);
+    g.sy = min(s.tile[1], s.nc[1] - g.y0);
+    g.sz = min(s.tile[2], s.nc[2] - g.z0);
+    g.ne = s.mode == MODE_WHOLE ? 1 : (g.sx + 2) * (g.sy + 2) * (g.sz + 2);
+    g.c0 = s.tile_cell0[t];
+    g.c1 = s.tile_cell0[t + 1];
+    return g;
+}
+
+__device__ __forceinline__ int wrap_cell(int c, int n) { return c < 0 ? c + n : (c >= n ? c - n : c); }
+
+enum { STAGE_RAW = 0, STAGE_REL = 1, STAGE_INDEX = 2 };
+
+// Stage the atoms of a tile's halo box (or of the whole system) into LDS, cell by cell
+// in halo order.  pref[e] is the first slot of halo cell e.  Slot SCAP is a NaN
+// sentinel (list padding points there; NaN fails every cutoff comparison).
+// Returns the number of staged atoms (uniform over the block); > SCAP means overflow
+// and nothing was staged.
+template <int SCAP, int WHAT>
+__device__ int stage_tile(const SysDev& s, const State& st, const TileGeom& g, double cx, double cy,
+                          double cz, double* Px, double* Py, double* Pz, uint8_t* Pt, int* pref,
+                          int* ecell, int* stmp)
+{
+    const int tid = threadIdx.x;
+    int cnt = 0;
+    const int ne = g.ne;
+    if (s.mode == MODE_WHOLE) {
+        if (tid == 0) {
+            ecell[0] = 0;
+            cnt = s.n;
+        }
+    } else if (tid < ne) {
+        const int hx = g.sx + 2, hy = g.sy + 2;
+        const int ex = tid % hx, ey = (tid / hx) % hy, ez = tid / (hx * hy);
+        const int cxn = wrap_cell(g.x0 - 1 + ex, s.nc[0]);
+        const int cyn = wrap_cell(g.y0 - 1 + ey, s.nc[1]);
+        const int czn = wrap_cell(g.z0 - 1 + ez, s.nc[2]);
+        const int sc = s.nat2sorted[(czn * s.nc[1] + cyn) * s.nc[0] + cxn];
+        const int b = s.cell_start[sc];
+        ecell[tid] = b;
+        cnt = s.cell_start[sc + 1] - b;
+    }
+    int total;
+    const int ex = block_excl_scan(cnt, stmp, &total);
+    if (tid < ne) pref[tid] = ex;
+    if (tid == 0) pref[ne] = total;
+    __syncthreads();
+    if (total > SCAP) return total;
+    for (int sl = tid; sl < total; sl += blockDim.x) {
+        int lo = 0, hi = ne - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pref[mid] <= sl)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int a = ecell[lo] + (sl - pref[lo]);
+        if (WHAT == STAGE_INDEX) {
+            reinterpret_cast<int*>(Px)[sl] = a;
+        } else {
+            double x = st.x[a], y = st.y[a], z = st.z[a];
+            if (WHAT == STAGE_REL) {
+                x = mi_fast(x - cx, s.L[0], s.Linv[0]);
+                y = mi_fast(y - cy, s.L[1], s.Linv[1]);
+                z = mi_fast(z - cz, s.L[2], s.Linv[2]);
+            }
+            Px[sl] = x;
+            Py[sl] = y;
+            Pz[sl] = z;
+            Pt[sl] = st.type[a];
+        }
+    }
+    if (tid == 0 && WHAT != STAGE_INDEX) {
+        Px[SCAP] = __longlong_as_double(0x7ff8000000000000ll);
+        Py[SCAP] = Px[SCAP];
+        Pz[SCAP] = Px[SCAP];
+        Pt[SCAP] = 0;
+    }
+    __syncthreads();
+    return total;
+}
+
+// Tile-local halo entry and LDS slot of own atom `a` (modes TILE_REL / RAW_MI).
+__device__ __forceinline__ void own_slot(const SysDev& s, const TileGeom& g, const int* pref, int a,
+                                         int* e0, int* slot)
+{
+    int lo = g.c0, hi = g.c1 - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s.cell_start[mid] <= a)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int k = lo - g.c0;
+    const int lx = k % g.sx, ly = (k / g.sx) % g.sy, lz = k / (g.sx * g.sy);
+    *e0 = ((lz + 1) * (g.sy + 2) + (ly + 1)) * (g.sx + 2) + (lx + 1);
+    *slot = pref[*e0] + (a - s.cell_start[lo]);
+}
+
+// ---------------------------------------------------------------------------------
+// state transfer
+
+__global__ void k_load_aos(const SysDev* __restrict__ sys, int k, const double* __restrict__ pos,
+                           const double* __restrict__ vel, const double* __restrict__ force,
+                           const uint8_t* __restrict__ type)
+{
+    const SysDev& s = sys[k];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    const State st = s.st[0];
+    st.x[i] = pos[3 * i];
+    st.y[i] = pos[3 * i + 1];
+    st.z[i] = pos[3 * i + 2];
+    st.vx[i] = vel ? vel[3 * i] : 0.0;
+    st.vy[i] = vel ? vel[3 * i + 1] : 0.0;
+    st.vz[i] = vel ? vel[3 * i + 2] : 0.0;
+    st.fx[i] = force ? force[3 * i] : 0.0;
+    st.fy[i] = force ? force[3 * i + 1] : 0.0;
+    st.fz[i] = force ? force[3 * i + 2] : 0.0;
+    st.type[i] = type[i];
+    st.orig[i] = (uint32_t)i;
+}
+
+__global__ void k_store_aos(const SysDev* __restrict__ sys, int k, double* __restrict__ pos,
+                            double* __restrict__ vel, double* __restrict__ force)
+{
+    const SysDev& s = sys[k];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= s.n) return;
+    const State st = s.st[s.ctl[CTL_CUR]];
+    const uint32_t o = st.orig[p];
+    if (pos) {
+        pos[3 * o] = st.x[p];
+        pos[3 * o + 1] = st.y[p];
+        pos[3 * o + 2] = st.z[p];
+    }
+    if (vel) {
+        vel[3 * o] = st.vx[p];
+        vel[3 * o + 1] = st.vy[p];
+        vel[3 * o + 2] = st.vz[p];
+    }
+    if (force) {
+        force[3 * o] = st.fx[p];
+        force[3 * o + 1] = st.fy[p];
+        force[3 * o + 2] = st.fz[p];
+    }
+}
+
+__global__ void k_set_ctl(const SysDev* __restrict__ sys, int nsys, int slot, int value)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nsys) sys[k].ctl[slot] = value;
+}
+
+// ---------------------------------------------------------------------------------
+// integrator, first half (potential.rs:16-22) + skin trigger
+
+__global__ __launch_bounds__(kAtomThreads) void k_integrate_pos(const SysDev* __restrict__ sys,
+                                                                 const WorkItem* __restrict__ work)
+{
+    const WorkItem w = work[blockIdx.x];
+    const SysDev& s = sys[w.sys];
+    const int a = w.idx + threadIdx.x;
+    if (a >= s.n) return;
+    const State st = s.st[s.ctl[CTL_CUR]];
+    const double m = s.mass[st.type[a]];
+    const double dt = s.dt, dt2 = s.dt2;
+    // a_t = F/m (current_acceleration, properties.rs:32-39); x += v dt + (a 0.5) dt^2
+    double x = st.x[a] + (st.vx[a] * dt + (st.fx[a] / m) * 0.5 * dt2);
+    double y = st.y[a] + (st.vy[a] * dt + (st.fy[a] / m) * 0.5 * dt2);
+    double z = st.z[a] + (st.vz[a] * dt + (st.fz[a] / m) * 0.5 * dt2);
+    // apply_boundary_conditions_pos (simulation_box.rs:29-42): s = h^-1 r; s -= floor(s); r = h s
+    double sx = x * s.Linv[0], sy = y * s.Linv[1], sz = z * s.Linv[2];
+    sx = sx - floor(sx);
+    sy = sy - floor(sy);
+    sz = sz - floor(sz);
+    x = s.L[0] * sx;
+    y = s.L[1] * sy;
+    z = s.L[2] * sz;
+    st.x[a] = x;
+    st.y[a] = y;
+    st.z[a] = z;
+    const double dx = mi_fast(x - s.rx[a], s.L[0], s.Linv[0]);
+    const double dy = mi_fast(y - s.ry[a], s.L[1], s.Linv[1]);
+    const double dz = mi_fast(z - s.rz[a], s.L[2], s.Linv[2]);
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (!(d2 <= s.half_skin2)) {  // also catches NaN
+        atomicOr(&s.ctl[CTL_REBUILD], 1);
+        if (d2 != d2) atomicOr(&s.ctl[CTL_NAN], 1);
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// rebuild: counting sort by cell, deterministic within a cell
+
+__global__ __launch_bounds__(kAtomThreads) void k_bin(const SysDev* __restrict__ sys,
+                                                       const WorkItem* __restrict__ work, int nwork)
+{
+    for (int b = blockIdx.x; b < nwork; b += gridDim.x) {
+        const WorkItem w = work[b];
+        const SysDev& s = sys[w.sys];
+        if (!s.ctl[CTL_REBUILD]) continue;
+        const int a = w.idx + threadIdx.x;
+        if (a >= s.n) continue;
+        const State st = s.st[s.ctl[CTL_CUR]];
+        // rcut_cells (neighbour_list.rs:17-34): c = floor(s n) as usize (saturating), % n
+        const double r[3] = {st.x[a], st.y[a], st.z[a]};
+        int c[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double f = floor((r[d] * s.Linv[d]) * (double)s.nc[d]);
+            const int v = f > 0.0 ? (f < 2.0e9 ? (int)f : 2000000000) : 0;
+            c[d] = v % s.nc[d];
+        }
+        const int sc = s.nat2sorted[(c[2] * s.nc[1] + c[1]) * s.nc[0] + c[0]];
+        s.atom_cell[a] = sc;
+        atomicAdd(&s.cell_count[sc], 1);
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_scan(const SysDev* __restrict__ sys)
+{
+    __shared__ int tmp[16];
+    const SysDev& s = sys[blockIdx.x];
+    if (!s.ctl[CTL_REBUILD]) return;
+    const int n = s.ncell;
+    const int per = (n + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int b = min(n, (int)threadIdx.x * per), e = min(n, b + per);
+    int sum = 0;
+    for (int i = b; i < e; ++i) sum += s.cell_count[i];
+    int total;
+    int run = block_excl_scan(sum, tmp, &total);
+    for (int i = b; i < e; ++i) {
+        const int c = s.cell_count[i];
+        s.cell_start[i] = run;
+        run += c;
+        s.cell_count[i] = 0;
+        s.cell_fill[i] = 0;
+    }
+    if (threadIdx.x == 0) s.cell_start[n] = total;
+}
+
+__global__ __launch_bounds__(kAtomThreads) void k_scatter(const SysDev* __restrict__ sys,
+                                                           const WorkItem* __restrict__ work, int nwork)
+{
+    for (int b = blockIdx.x; b < nwork; b += gridDim.x) {
+        const WorkItem w = work[b];
+        const SysDev& s = sys[w.sys];
+        if (!s.ctl[CTL_REBUILD]) continue;
+        const int a = w.idx + threadIdx.x;
+        if (a >= s.n) continue;
+        const State st = s.st[s.ctl[CTL_CUR]];
+        const int sc = s.atom_cell[a];
+        const int p = s.cell_start[sc] + atomicAdd(&s.cell_fill[sc], 1);
+        s.scat_src[p] = a;
+        s.scat_key[p] = st.orig[a];
+    }
+}
+
+// Inside a cell, atoms end up in ascending original index (the reference's rcut_cells
+// order), whatever order the atomics placed them in.
+__global__ __launch_bounds__(kAtomThreads) void k_rank_permute(const SysDev* __restrict__ sys,
+                                                                const WorkItem* __restrict__ work,
+                                                                int nwork)
+{
+    for (int b = blockIdx.x; b < nwork; b += gridDim.x) {
+        const WorkItem w = work[b];
+        const SysDev& s = sys[w.sys];
+        if (!s.ctl[CTL_REBUILD]) continue;
+        const int p = w.idx + threadIdx.x;
+        if (p >= s.n) continue;
+        const int cur = s.ctl[CTL_CUR];
+        const State src = s.st[cur], dst = s.st[cur ^ 1];
+        const int a = s.scat_src[p];
+        const int sc = s.atom_cell[a];
+        const int c0 = s.cell_start[sc], c1 = s.cell_start[sc + 1];
+        const uint32_t key = s.scat_key[p];
+        int rank = 0;
+        for (int q = c0; q < c1; ++q) rank += s.scat_key[q] < key;
+        const int np = c0 + rank;
+        const double x = src.x[a], y = src.y[a], z = src.z[a];
+        dst.x[np] = x;
+        dst.y[np] = y;
+        dst.z[np] = z;
+        s.rx[np] = x;
+        s.ry[np] = y;
+        s.rz[np] = z;
+        dst.vx[np] = src.vx[a];
+        dst.vy[np] = src.vy[a];
+        dst.vz[np] = src.vz[a];
+        dst.fx[np] = src.fx[a];
+        dst.fy[np] = src.fy[a];
+        dst.fz[np] = src.fz[a];
+        dst.type[np] = src.type[a];
+        dst.orig[np] = key;
+    }
+}
+
+// Full neighbour list at rcut_ij + skin.  Predicate exactly as build_neighbour_list
+// (lennard_jones.rs:398-405): rij = r_j - r_i, minimum image with round(), and
+// `norm() > rcut` rejects -- evaluated as r2 > r2_list where r2_list is the largest
+// double whose correctly rounded sqrt is <= rcut + skin, and r2 = (x^2+y^2)+z^2 rounded
+// like nalgebra's norm_squared.  Entries are LDS byte offsets of the tile's staging.
+template <int SCAP>
+__global__ __launch_bounds__(kTileThreads) void k_build_list(const SysDev* __restrict__ sys,
+                                                              const WorkItem* __restrict__ tiles,
+                                                              int ntw)
+{
+    __shared__ double Px[SCAP + 1], Py[SCAP + 1], Pz[SCAP + 1];
+    __shared__ uint8_t Pt[SCAP + 1];
+    __shared__ int pref[kMaxHalo + 1];
+    __shared__ int ecell[kMaxHalo];
+    __shared__ int stmp[16];
+    __shared__ double r2l[kMaxTypes * kMaxTypes];
+    for (int b = blockIdx.x; b < ntw; b += gridDim.x) {
+        const WorkItem w = tiles[xcd_remap(b, ntw)];
+        const SysDev& s = sys[w.sys];
+        if (!s.ctl[CTL_REBUILD]) continue;
+        const State st = s.st[s.ctl[CTL_CUR] ^ 1];
+        const TileGeom g = tile_geom(s, w.idx);
+        const int nt = s.ntypes;
+        for (int q = threadIdx.x; q < nt * nt; q += blockDim.x) r2l[q] = s.pp[q].r2_list;
+        const int total =
+            stage_tile<SCAP, STAGE_RAW>(s, st, g, 0.0, 0.0, 0.0, Px, Py, Pz, Pt, pref, ecell, stmp);
+        if (total > SCAP) {
+            if (threadIdx.x == 0) atomicOr(&s.ctl[CTL_OVF_STAGE], 1);
+            __syncthreads();
+            continue;
+        }
+        const int a0 = s.cell_start[g.c0], a1 = s.cell_start[g.c1];
+        const int hx = g.sx + 2, hxy = (g.sx + 2) * (g.sy + 2);
+        int mymax = 0;
+        for (int a = a0 + (int)threadIdx.x; a < a1; a += blockDim.x) {
+            const double xi = st.x[a], yi = st.y[a], zi = st.z[a];
+            const int ti = st.type[a];
+            int e0 = 0, self;
+            if (s.mode == MODE_WHOLE)
+                self = a;
+            else
+                own_slot(s, g, pref, a, &e0, &self);
+            uint64_t lo = 0, hi = 0;
+            int cnt = 0;
+            const int maxg = s.maxnb >> 3;
+            uint4* out = s.nbr + a;
+            auto push = [&](uint32_t off) {
+                lo = (lo >> 16) | (hi << 48);
+                hi = (hi >> 16) | ((uint64_t)off << 48);
+                ++cnt;
+                if ((cnt & 7) == 0 && (cnt >> 3) <= maxg)
+                    out[(size_t)((cnt >> 3) - 1) * s.npad] =
+                        make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+            };
+            auto test = [&](int sl) {
+                if (sl == self) return;
+                const double dx = mi_ref(Px[sl] - xi, s.L[0], s.Linv[0]);
+                const double dy = mi_ref(Py[sl] - yi, s.L[1], s.Linv[1]);
+                const double dz = mi_ref(Pz[sl] - zi, s.L[2], s.Linv[2]);
+                const double r2 = (dx * dx + dy * dy) + dz * dz;
+                if (r2 <= r2l[ti * nt + Pt[sl]]) push((uint32_t)sl << 3);
+            };
+            if (s.mode == MODE_WHOLE) {
+                for (int sl = 0; sl < total; ++sl) test(sl);
+            } else {
+                for (int dz = -1; dz <= 1; ++dz)
+                    for (int dy = -1; dy <= 1; ++dy) {
+                        const int e = e0 + dy * hx + dz * hxy;
+                        const int sb = pref[e - 1], se = pref[e + 2];
+                        for (int sl = sb; sl < se; ++sl) test(sl);
+                    }
+            }
+            const int n_real = cnt;
+            if (cnt & 7) {
+                while (cnt & 7) {
+                    lo = (lo >> 16) | (hi << 48);
+                    hi = (hi >> 16) | ((uint64_t)(SCAP << 3) << 48);
+                    ++cnt;
+                }
+                if ((cnt >> 3) <= maxg)
+                    out[(size_t)((cnt >> 3) - 1) * s.npad] =
+                        make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+            }
+            s.nnb[a] = (uint16_t)min(n_real, s.maxnb);
+            mymax = max(mymax, n_real);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) mymax = max(mymax, __shfl_xor(mymax, o, 64));
+        if ((threadIdx.x & 63) == 0 && mymax > 0) {
+            atomicMax(&s.ctl[CTL_MAXNB], mymax);
+            if (mymax > s.maxnb) atomicOr(&s.ctl[CTL_OVF_NB], 1);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// pair force
+
+struct Acc {
+    double fx, fy, fz, pe, wp;
+};
+
+// One candidate.  FP64: LennardJones::compute_potential (lennard_jones.rs:32-55) with a
+// refined hardware reciprocal for 1/r2.  MIXED: the cutoff test and r2 stay FP64 (so the
+// pair set is the FP64 one), the LJ arithmetic is FP32, and the results are accumulated
+// in FP64.
+template <bool MIXED>
+__device__ __forceinline__ void lj_eval(const PairParam& p, double dx, double dy, double dz, Acc& acc)
+{
+    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    if (r2 <= p.r2_force) {
+        if (!MIXED) {
+            double inv = __builtin_amdgcn_rcp(r2);
+            inv = fma(inv, fma(-r2, inv, 1.0), inv);
+            inv = fma(inv, fma(-r2, inv, 1.0), inv);
+            const double sr2 = p.sig2 * inv;
+            const double sr6 = sr2 * sr2 * sr2;
+            const double fp = fma(p.c48, sr6, -p.c24) * sr6 * inv;
+            acc.fx = fma(-fp, dx, acc.fx);
+            acc.fy = fma(-fp, dy, acc.fy);
+            acc.fz = fma(-fp, dz, acc.fz);
+            acc.pe += fma(fma(p.c4, sr6, -p.c4), sr6, -p.ucut);
+            acc.wp = fma(fp, r2, acc.wp);
+        } else {
+            const float r2f = (float)r2;
+            const float inv = __builtin_amdgcn_rcpf(r2f);
+            const float sr2 = p.sig2f * inv;
+            const float sr6 = sr2 * sr2 * sr2;
+            const float fpf = fmaf(p.c48f, sr6, -p.c24f) * sr6 * inv;
+            const float uf = fmaf(fmaf(p.c4f, sr6, -p.c4f), sr6, -p.ucutf);
+            const double fp = (double)fpf;
+            acc.fx = fma(-fp, dx, acc.fx);
+            acc.fy = fma(-fp, dy, acc.fy);
+            acc.fz = fma(-fp, dz, acc.fz);
+            acc.pe += (double)uf;
+            acc.wp = fma(fp, r2, acc.wp);
+        }
+    }
+}
+
+template <int SCAP, bool MIXED, bool MULTI, bool PAIRMI>
+__device__ __forceinline__ void atom_forces(const SysDev& s, const double* Px, const double* Py,
+                                            const double* Pz, const uint8_t* Pt,
+                                            const PairParam* ppl, const PairParam& p0, int a, int ti,
+                                            double qx, double qy, double qz, Acc& acc)
+{
+    const int nn = s.nnb[a];
+    const int ng = (nn + 7) >> 3;
+    const uint4* L = s.nbr + a;
+    const char* bx = reinterpret_cast<const char*>(Px);
+    const char* by = reinterpret_cast<const char*>(Py);
+    const char* bz = reinterpret_cast<const char*>(Pz);
+    for (int gi = 0; gi < ng; ++gi) {
+        const uint4 u = L[(size_t)gi * s.npad];
+        const uint32_t wv[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const uint32_t off = (wv[e >> 1] >> ((e & 1) * 16)) & 0xffffu;
+            double dx = *reinterpret_cast<const double*>(bx + off) - qx;
+            double dy = *reinterpret_cast<const double*>(by + off) - qy;
+            double dz = *reinterpret_cast<const double*>(bz + off) - qz;
+            if (PAIRMI) {
+                dx = mi_fast(dx, s.L[0], s.Linv[0]);
+                dy = mi_fast(dy, s.L[1], s.Linv[1]);
+                dz = mi_fast(dz, s.L[2], s.Linv[2]);
+            }
+            if (MULTI)
+                lj_eval<MIXED>(ppl[ti * s.ntypes + Pt[off >> 3]], dx, dy, dz, acc);
+            else
+                lj_eval<MIXED>(p0, dx, dy, dz, acc);
+        }
+    }
+}
+
+template <int SCAP, bool MIXED>
+__global__ __launch_bounds__(kTileThreads) void k_force(const SysDev* __restrict__ sys,
+                                                         const WorkItem* __restrict__ tiles, int ntw,
+                                                         int integrate)
+{
+    __shared__ double Px[SCAP + 1], Py[SCAP + 1], Pz[SCAP + 1];
+    __shared__ uint8_t Pt[SCAP + 1];
+    __shared__ int pref[kMaxHalo + 1];
+    __shared__ int ecell[kMaxHalo];
+    __shared__ int stmp[16];
+    __shared__ double rtmp[16];
+    __shared__ PairParam ppl[kMaxTypes * kMaxTypes];
+
+    const WorkItem w = tiles[xcd_remap(blockIdx.x, ntw)];
+    const SysDev& s = sys[w.sys];
+    const State st = s.st[s.ctl[CTL_CUR] ^ (s.ctl[CTL_REBUILD] ? 1 : 0)];
+    const TileGeom g = tile_geom(s, w.idx);
+    const bool rel = s.mode == MODE_TILE_REL;
+    const double cx = (g.x0 + 0.5 * g.sx) * s.cellw[0];
+    const double cy = (g.y0 + 0.5 * g.sy) * s.cellw[1];
+    const double cz = (g.z0 + 0.5 * g.sz) * s.cellw[2];
+    const int nt = s.ntypes;
+    for (int q = threadIdx.x; q < nt * nt; q += blockDim.x) ppl[q] = s.pp[q];
+    int total;
+    if (rel)
+        total = stage_tile<SCAP, STAGE_REL>(s, st, g, cx, cy, cz, Px, Py, Pz, Pt, pref, ecell, stmp);
+    else
+        total = stage_tile<SCAP, STAGE_RAW>(s, st, g, cx, cy, cz, Px, Py, Pz, Pt, pref, ecell, stmp);
+    double sum[4] = {0.0, 0.0, 0.0, 0.0};
+    if (total <= SCAP) {
+        const PairParam p0 = ppl[0];
+        const int a0 = s.cell_start[g.c0], a1 = s.cell_start[g.c1];
+        for (int a = a0 + (int)threadIdx.x; a < a1; a += blockDim.x) {
+            const double xi = st.x[a], yi = st.y[a], zi = st.z[a];
+            const int ti = st.type[a];
+            double qx = xi, qy = yi, qz = zi;
+            if (rel) {
+                qx = mi_fast(xi - cx, s.L[0], s.Linv[0]);
+                qy = mi_fast(yi - cy, s.L[1], s.Linv[1]);
+                qz = mi_fast(zi - cz, s.L[2], s.Linv[2]);
+            }
+            Acc acc = {0.0, 0.0, 0.0, 0.0, 0.0};
+            if (nt == 1) {
+                if (rel)
+                    atom_forces<SCAP, MIXED, false, false>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, acc);
+                else
+                    atom_forces<SCAP, MIXED, false, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, acc);
+            } else {
+                if (rel)
+                    atom_forces<SCAP, MIXED, true, false>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, acc);
+                else
+                    atom_forces<SCAP, MIXED, true, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, acc);
+            }
+            const double m = s.mass[ti];
+            double vx = st.vx[a], vy = st.vy[a], vz = st.vz[a];
+            if (integrate) {
+                // v += ((a_t + a_{t+dt}) 0.5) dt  (potential.rs:28-30)
+                const double dt = s.dt;
+                vx = vx + ((st.fx[a] / m + acc.fx / m) * 0.5) * dt;
+                vy = vy + ((st.fy[a] / m + acc.fy / m) * 0.5) * dt;
+                vz = vz + ((st.fz[a] / m + acc.fz / m) * 0.5) * dt;
+                st.vx[a] = vx;
+                st.vy[a] = vy;
+                st.vz[a] = vz;
+            }
+            st.fx[a] = acc.fx;
+            st.fy[a] = acc.fy;
+            st.fz[a] = acc.fz;
+            sum[0] += acc.pe;
+            sum[1] += 0.5 * m * ((vx * vx + vy * vy) + vz * vz);
+            sum[2] += (xi * acc.fx + yi * acc.fy) + zi * acc.fz;
+            sum[3] += acc.wp;
+        }
+    } else if (threadIdx.x == 0) {
+        atomicOr(&s.ctl[CTL_OVF_STAGE], 1);
+    }
+    block_sum4(sum, rtmp);
+    if (threadIdx.x == 0) {
+        double* P = s.partial + 4 * (size_t)w.idx;
+        P[0] = sum[0];
+        P[1] = sum[1];
+        P[2] = sum[2];
+        P[3] = sum[3];
+    }
+}
+
+// Per-system fixed-order sum of the tile partials -> thermo record; optionally flips the
+// live buffer set after a rebuild.
+__global__ __launch_bounds__(256) void k_reduce(const SysDev* __restrict__ sys, int flip, int record)
+{
+    __shared__ double rtmp[16];
+    const SysDev& s = sys[blockIdx.x];
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int t = threadIdx.x; t < s.ntile; t += blockDim.x)
+        for (int k = 0; k < 4; ++k) v[k] += s.partial[4 * (size_t)t + k];
+    block_sum4(v, rtmp);
+    if (threadIdx.x != 0) return;
+    const double pe = 0.5 * v[0], ke = v[1], wabs = v[2], wpair = 0.5 * v[3];
+    if (record) {
+        const int k = s.ctl[CTL_STEP];
+        if (k < s.thermo_cap) {
+            ThermoDev t;
+            t.pe = pe;
+            t.ke = ke;
+            t.etot = pe + ke;
+            t.temperature = (2.0 * ke) / ((double)(3 * (int64_t)s.n) * s.kb);
+            t.pressure = (2.0 * ke + wabs) / (3.0 * s.volume);
+            t.virial_abs = wabs;
+            t.virial_pair = wpair;
+            s.thermo[k] = t;
+        }
+        s.ctl[CTL_STEP] = k + 1;
+    }
+    if (!isfinite(pe) || !isfinite(ke)) s.ctl[CTL_NAN] = 1;
+    if (flip && s.ctl[CTL_REBUILD]) {
+        s.ctl[CTL_CUR] ^= 1;
+        s.ctl[CTL_REBUILD] = 0;
+        s.ctl[CTL_NREBUILD] += 1;
+    }
+}
+
+__global__ void k_flip(const SysDev* __restrict__ sys, int nsys)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nsys) return;
+    const SysDev& s = sys[k];
+    if (s.ctl[CTL_REBUILD]) {
+        s.ctl[CTL_CUR] ^= 1;
+        s.ctl[CTL_REBUILD] = 0;
+        s.ctl[CTL_NREBUILD] += 1;
+    }
+}
+
+// Test support: the list of every atom as original indices, out[orig * maxnb + k].
+template <int SCAP>
+__global__ __launch_bounds__(kTileThreads) void k_list_to_orig(const SysDev* __restrict__ sys,
+                                                                const WorkItem* __restrict__ tiles,
+                                                                int ntw, uint32_t* __restrict__ out,
+                                                                uint32_t* __restrict__ counts)
+{
+    __shared__ double Px[SCAP + 1];
+    __shared__ int pref[kMaxHalo + 1];
+    __shared__ int ecell[kMaxHalo];
+    __shared__ int stmp[16];
+    const WorkItem w = tiles[blockIdx.x];
+    const SysDev& s = sys[w.sys];
+    const State st = s.st[s.ctl[CTL_CUR]];
+    const TileGeom g = tile_geom(s, w.idx);
+    const int total = stage_tile<SCAP, STAGE_INDEX>(s, st, g, 0.0, 0.0, 0.0, Px, nullptr, nullptr,
+                                                    nullptr, pref, ecell, stmp);
+    if (total > SCAP) return;
+    const int* Li = reinterpret_cast<const int*>(Px);
+    const int a0 = s.cell_start[g.c0], a1 = s.cell_start[g.c1];
+    for (int a = a0 + (int)threadIdx.x; a < a1; a += blockDim.x) {
+        const uint32_t oa = st.orig[a];
+        const int nn = s.nnb[a];
+        for (int k = 0; k < nn; ++k) {
+            const uint4 u = s.nbr[(size_t)(k >> 3) * s.npad + a];
+            const uint32_t wv[4] = {u.x, u.y, u.z, u.w};
+            const uint32_t off = (wv[(k & 7) >> 1] >> (((k & 7) & 1) * 16)) & 0xffffu;
+            out[(size_t)oa * s.maxnb + k] = st.orig[Li[off >> 3]];
+        }
+        counts[oa] = (uint32_t)nn;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// launch wrappers
+
+// Staging capacities with compiled kernel instances.
+#define PIS_SCAP_DISPATCH(scap, M) \
+    do {                            \
+        if ((scap) <= 1024)         \
+            M(1024);                \
+        else if ((scap) <= 1536)    \
+            M(1536);                \
+        else if ((scap) <= 2048)    \
+            M(2048);                \
+        else                        \
+            M(4096);                \
+    } while (0)
+
+static inline int grid_cap(int n, int cap) { return n < cap ? (n > 0 ? n : 1) : cap; }
+
+void launch_load_aos(hipStream_t st, const SysDev* d_sys, int sys, int n, const double* pos,
+                     const double* vel, const double* force, const uint8_t* type)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_load_aos, dim3((n + 255) / 256), dim3(256), 0, st, d_sys, sys, pos, vel,
+                       force, type);
+}
+
+void launch_store_aos(hipStream_t st, const SysDev* d_sys, int sys, int n, double* pos, double* vel,
+                      double* force)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_store_aos, dim3((n + 255) / 256), dim3(256), 0, st, d_sys, sys, pos, vel,
+                       force);
+}
+
+void launch_set_ctl(hipStream_t st, const SysDev* d_sys, int nsys, int slot, int value)
+{
+    hipLaunchKernelGGL(k_set_ctl, dim3((nsys + 255) / 256), dim3(256), 0, st, d_sys, nsys, slot, value);
+}
+
+void launch_integrate_pos(hipStream_t st, const SysDev* d_sys, const WorkItem* atom_work,
+                          int n_atom_work)
+{
+    hipLaunchKernelGGL(k_integrate_pos, dim3(n_atom_work), dim3(kAtomThreads), 0, st, d_sys, atom_work);
+}
+
+void launch_rebuild(hipStream_t st, const SysDev* d_sys, int nsys, const WorkItem* atom_work,
+                    int n_atom_work, const WorkItem* tile_work, int n_tile_work, int scap,
+                    bool multi_type)
+{
+    (void)multi_type;
+    const int ga = grid_cap(n_atom_work, 2048);
+    hipLaunchKernelGGL(k_bin, dim3(ga), dim3(kAtomThreads), 0, st, d_sys, atom_work, n_atom_work);
+    hipLaunchKernelGGL(k_scan, dim3(nsys), dim3(1024), 0, st, d_sys);
+    hipLaunchKernelGGL(k_scatter, dim3(ga), dim3(kAtomThreads), 0, st, d_sys, atom_work, n_atom_work);
+    hipLaunchKernelGGL(k_rank_permute, dim3(ga), dim3(kAtomThreads), 0, st, d_sys, atom_work,
+                       n_atom_work);
+    const int gt = grid_cap(n_tile_work, 2048);
+#define PIS_BUILD(S) \
+    hipLaunchKernelGGL(k_build_list<S>, dim3(gt), dim3(kTileThreads), 0, st, d_sys, tile_work, n_tile_work)
+    PIS_SCAP_DISPATCH(scap, PIS_BUILD);
+#undef PIS_BUILD
+}
+
+void launch_force(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work, int n_tile_work,
+                  int scap, bool multi_type, bool mixed, int integrate)
+{
+    (void)multi_type;
+    const dim3 g(n_tile_work), b(kTileThreads);
+#define PIS_FORCE_M(S) \
+    hipLaunchKernelGGL((k_force<S, true>), g, b, 0, st, d_sys, tile_work, n_tile_work, integrate)
+#define PIS_FORCE_D(S) \
+    hipLaunchKernelGGL((k_force<S, false>), g, b, 0, st, d_sys, tile_work, n_tile_work, integrate)
+    if (mixed)
+        PIS_SCAP_DISPATCH(scap, PIS_FORCE_M);
+    else
+        PIS_SCAP_DISPATCH(scap, PIS_FORCE_D);
+#undef PIS_FORCE_M
+#undef PIS_FORCE_D
+}
+
+void launch_reduce(hipStream_t st, const SysDev* d_sys, int nsys, int flip, int record)
+{
+    hipLaunchKernelGGL(k_reduce, dim3(nsys), dim3(256), 0, st, d_sys, flip, record);
+}
+
+void launch_flip(hipStream_t st, const SysDev* d_sys, int nsys)
+{
+    hipLaunchKernelGGL(k_flip, dim3((nsys + 255) / 256), dim3(256), 0, st, d_sys, nsys);
+}
+
+void launch_list_to_orig(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work,
+                         int n_tile_work, int scap, bool multi_type, uint32_t* out, uint32_t* counts)
+{
+    (void)multi_type;
+    const dim3 g(n_tile_work), b(kTileThreads);
+#define PIS_L2O(S) \
+    hipLaunchKernelGGL(k_list_to_orig<S>, g, b, 0, st, d_sys, tile_work, n_tile_work, out, counts)
+    PIS_SCAP_DISPATCH(scap, PIS_L2O);
+#undef PIS_L2O
+}
+
+size_t tile_lds_bytes(int scap, bool multi_type)
+{
+    (void)multi_type;
+    return (size_t)(scap + 1) * 25 + (kMaxHalo * 2 + 1) * 4;
+}
+
+}  // namespace pis

@@ -1,0 +1,165 @@
+"""GPU parity: the gfx950 path through the C ABI against the CPU oracle.
+
+Bars (from the project's correctness contract):
+  * neighbour pair set bit-identical to build_neighbour_list (lennard_jones.rs:344-416)
+    with every rcut raised by the skin (brute-force LJManager semantics when a box side
+    holds fewer than 3 cells, where the reference's stencil double counts);
+  * FP64 per-atom forces within 1e-10 of LJVOffsetManager (lennard_jones.rs:186-244),
+    measured normwise max_i|dF_i| / RMS|F|; PE within 1e-12 of an exactly rounded sum;
+  * the FP64 path tracks the oracle's NVE energies over 1000 steps (C1 argon crystal).
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from pis_amd import Engine, PisError, workloads as W
+from tests.conftest import force_err, force_rms_err
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_forces(spec, precision="fp64"):
+    with Engine(spec, precision) as eng:
+        eng.load()
+        pe = float(eng.compute_potential()[0])
+        _, _, f = eng.get_state(0)
+    return pe, f
+
+
+def oracle_forces(spec, manager=po.MGR_OFFSET):
+    o = po.OSystem.from_spec(spec)
+    pe = o.compute(manager)
+    return pe, o.force.copy()
+
+
+def check_pe(pe, spec, pe_serial):
+    """PE against the oracle two ways.  (1) 1e-12 relative to an exactly rounded sum
+    (math.fsum of the oracle's per-atom energies, halved): the GPU's fixed-order tree
+    sum has nothing but rounding to answer for.  (2) Against the reference's own serial
+    half-list sum (lennard_jones.rs:229), within that sum's recursive-summation error
+    bound n*u*sum|U| -- on ordered lattices its rounding errors do not cancel."""
+    import math
+
+    o = po.OSystem.from_spec(spec)
+    _, pe_atom = o.compute_fulllist(0.0, True, brute=True)
+    exact = math.fsum(pe_atom) / 2.0
+    assert abs(pe - exact) <= 1e-12 * abs(exact)
+    st, _ = o.list(0.0, brute=True)
+    nterms = int(st[-1]) // 2
+    bound = nterms * 2.0 ** -53 * (np.abs(pe_atom).sum() / 2.0)
+    assert abs(pe - pe_serial) <= max(bound, 1e-12 * abs(exact))
+
+
+SPECS = {
+    "lj_liquid_4000": lambda: W.lj_liquid(10, 0.8442, 0.72, seed=3),
+    "argon_c1_jittered": lambda: _jittered_argon(),
+    "lj_rc3_6912": lambda: W.lj_liquid(12, 0.8442, 1.0, seed=5, rcut=3.0),
+    "ka_2048": lambda: W.ka_mixture(2048, 1.2, 1.0, seed=9),
+    "ka_5000": lambda: W.ka_mixture(5000, 1.2, 0.7, seed=10),
+    "lj_dilute_3000": lambda: W.lj_box(3000, 0.3, 1.5, seed=4),
+}
+
+
+def _jittered_argon():
+    s = W.argon_fcc_c1()
+    rng = np.random.default_rng(77)
+    s.positions = W.wrap_positions(s.positions + rng.uniform(-0.1, 0.1, s.positions.shape), s.box.lengths)
+    return s
+
+
+@pytest.mark.parametrize("name", sorted(SPECS))
+def test_fp64_forces_match_offset_manager(name):
+    spec = SPECS[name]()
+    pe, f = gpu_forces(spec)
+    pe_ref, f_ref = oracle_forces(spec)
+    assert force_err(f, f_ref) <= 1e-10
+    check_pe(pe, spec, pe_ref)
+
+
+@pytest.mark.parametrize("name", sorted(SPECS))
+def test_pair_set_bit_identical(name):
+    spec = SPECS[name]()
+    with Engine(spec, "fp64") as eng:
+        eng.load()
+        off, idx = eng.neighbor_list()
+    o = po.OSystem.from_spec(spec)
+    st, ix = o.list(rcut_add=spec.skin)
+    got = po.csr_to_sorted_pairs(off, idx)
+    want = po.csr_to_sorted_pairs(st, ix)
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("n", [256, 500])
+def test_small_box_pair_set_matches_brute_force(n):
+    spec = W.ka_mixture(n, 1.2, 1.0, seed=n)
+    with Engine(spec, "fp64") as eng:
+        eng.load()
+        off, idx = eng.neighbor_list()
+        pe = float(eng.compute_potential()[0])
+        _, _, f = eng.get_state(0)
+    o = po.OSystem.from_spec(spec)
+    st, ix = o.list(rcut_add=spec.skin, brute=True)
+    assert np.array_equal(po.csr_to_sorted_pairs(off, idx), po.csr_to_sorted_pairs(st, ix))
+    pe_ref, f_ref = oracle_forces(spec, po.MGR_BRUTE)
+    assert force_err(f, f_ref) <= 1e-10
+    check_pe(pe, spec, pe_ref)
+
+
+def test_mixed_force_rms_error():
+    spec = W.lj_liquid(10, 0.8442, 0.72, seed=3)
+    _, f = gpu_forces(spec, "mixed")
+    _, f_ref = oracle_forces(spec)
+    assert force_rms_err(f, f_ref) <= 1e-5
+
+
+def test_argon_c1_tracks_oracle_energies():
+    spec = W.argon_fcc_c1()
+    nsteps = 1000
+    with Engine(spec, "fp64") as eng:
+        eng.load()
+        eng.compute_potential()
+        th = eng.run_nve(spec.dt, nsteps)[:, 0, :]
+    o = po.OSystem.from_spec(spec)
+    ref = o.run_nve(spec.dt, nsteps, po.MGR_OFFSET, kb=spec.kb)
+    scale = np.abs(ref[:, 0]).max()
+    assert np.abs(th[:, 0] - ref[:, 0]).max() <= 1e-9 * scale  # PE
+    assert np.abs(th[:, 1] - ref[:, 1]).max() <= 1e-9 * scale  # KE
+    assert np.abs(th[:, 2] - ref[:, 2]).max() <= 1e-9 * scale  # H
+    assert np.abs(th[:, 3] - ref[:, 3]).max() <= 1e-7 * np.abs(ref[:, 3]).max()  # T
+
+
+def test_repeat_runs_bit_identical():
+    spec = W.lj_liquid(10, 0.8442, 0.72, seed=8)
+    out = []
+    for _ in range(2):
+        with Engine(spec, "fp64") as eng:
+            eng.load()
+            eng.compute_potential()
+            th = eng.run_nve(spec.dt, 200)
+            out.append((th, eng.get_state(0)))
+    assert np.array_equal(out[0][0], out[1][0])
+    for a, b in zip(out[0][1], out[1][1]):
+        assert np.array_equal(a, b)
+
+
+def test_errors_map_to_status():
+    spec = W.lj_liquid(4, 0.8442, 0.72, seed=1)
+    bad = W.lj_liquid(4, 0.8442, 0.72, seed=1)
+    bad.type_ids = bad.type_ids.copy()
+    bad.type_ids[3] = 5
+    with pytest.raises(PisError) as e:
+        Engine(bad)
+    assert e.value.status == 4
+    nopair = W.lj_liquid(4, 0.8442, 0.72, seed=1)
+    nopair.pairs = []
+    with pytest.raises(PisError) as e:
+        Engine(nopair)
+    assert e.value.status == 2
+    with Engine(spec) as eng:
+        pos = spec.positions.copy()
+        pos[1] = pos[0]  # coincident atoms -> infinite energy
+        eng.set_state(0, pos, spec.velocities)
+        with pytest.raises(PisError) as e:
+            eng.compute_potential()
+        assert e.value.status == 3
