@@ -1,0 +1,239 @@
+#!/usr/bin/env python3
+"""Benchmark of the LJ force + velocity-Verlet hot path on one MI355X.
+
+Default workload: LJ liquid, 1,048,576 atoms (64^3 fcc cells at reduced density 0.8442,
++-0.05 sigma seeded jitter), T* = 0.72, rc = 2.5 sigma, skin 0.3 sigma, dt = 0.005, NVE.
+A "step" is one verlet_step_nve of the whole system with state resident in HBM
+(position update + wrap + skin check, list rebuild when triggered, LJ forces with the
+fused velocity update, per-step PE/KE/virial reduction -- the reference computes thermo
+every step).  The headline value is the mixed-precision rate (FP32 pair arithmetic,
+FP64 state); the plain FP64 path is timed on the same workload beside it.
+
+Prints ONE JSON line.  Only one GPU is used; --gpus above 1 exits with a message.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "atom-steps/s at 1M LJ atoms, FP64 vs FP32/FP64 mixed; p50/p99 job latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def workload(name):
+    from pis_amd import workloads as W
+
+    if name == "c2":
+        spec = W.lj_liquid(64, 0.8442, 0.72, seed=2026, rcut=2.5, skin=0.3, dt=0.005)
+        desc = ("LJ liquid 1,048,576 atoms (64^3 fcc, rho*=0.8442, +-0.05 sigma jitter), T*=0.72, "
+                "rc=2.5 sigma, skin 0.3 sigma, dt=0.005, NVE velocity-Verlet")
+    elif name == "c3":
+        spec = W.lj_liquid(100, 0.8442, 1.0, seed=2027, rcut=3.0, skin=0.3, dt=0.005)
+        desc = ("LJ liquid 4,000,000 atoms (100^3 fcc, rho*=0.8442), T*=1.0, rc=3.0 sigma, "
+                "skin 0.3 sigma, dt=0.005, NVE velocity-Verlet")
+    else:
+        raise SystemExit(f"unknown config {name}")
+    return spec, desc
+
+
+def list_mean(spec):
+    rcl = max(p[4] for p in spec.pairs) + spec.skin
+    rho = spec.n / spec.box.volume()
+    return 4.0 / 3.0 * math.pi * rcl ** 3 * rho
+
+
+def force_kernel_bytes_per_atom(nbar):
+    """Algorithmic HBM bytes of one force launch per atom: the u16 list entries
+    (2 B each) and their count (2 B), position (24 B) and type (1 B) of the atom, old
+    force and velocity read (48 B), new force and velocity written (48 B).  Neighbour
+    positions are staged once per tile from L2/MALL and counted at zero, like the
+    SURVEY's step model does."""
+    return 2.0 * nbar + 2 + 24 + 1 + 48 + 48
+
+
+def step_bytes_per_atom(nbar):
+    """Whole step: force launch + position update (x, v, F, type, x_ref read; x written)."""
+    return force_kernel_bytes_per_atom(nbar) + (24 * 3 + 1 + 24 + 24)
+
+
+def drift_per_atom_per_1e4(etot, n):
+    t = np.arange(etot.shape[0], dtype=np.float64)
+    slope = np.polyfit(t, etot - etot[0], 1)[0]
+    return float(slope * 1e4 / n)
+
+
+def run_gpu(spec, precision, steps, warmup):
+    from pis_amd import Engine
+
+    eng = Engine(spec, precision)
+    eng.load()
+    eng.compute_potential()
+    _, _, f0 = eng.get_state(0)
+    if warmup:
+        eng.run_nve(spec.dt, warmup, record=False)
+    eng.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    th = eng.run_nve(spec.dt, steps, record=True)[:, 0, :]
+    t1 = time.perf_counter()
+    st = eng.stats()
+    eng.close()
+    sec = t1 - t0
+    return {
+        "sec": sec,
+        "rate": spec.n * steps / sec,
+        "ms_per_step": 1e3 * sec / steps,
+        "force_ms": st["force_kernel_ms"] / max(st["timed_force_launches"], 1),
+        "rebuilds": st["rebuilds"],
+        "max_neighbors": st["max_neighbors"],
+        "thermo": th,
+        "f0": f0,
+    }
+
+
+def cpu_baseline(spec, steps):
+    """The oracle (faithful LJVOffsetManager + verlet_step_nve, 1 core) on `steps` steps
+    of the same system; also returns its initial forces for the force-error figures."""
+    from oracle import pyoracle as po
+
+    o = po.OSystem.from_spec(spec)
+    t0 = time.perf_counter()
+    o.compute(po.MGR_OFFSET)
+    t1 = time.perf_counter()
+    f_ref = o.force.copy()
+    t2 = time.perf_counter()
+    for _ in range(steps):
+        o.verlet_step_nve(spec.dt, po.MGR_OFFSET)
+        o.kinetic_energy()
+    t3 = time.perf_counter()
+    return f_ref, spec.n * steps / (t3 - t2), t1 - t0, t3 - t2
+
+
+def cpu_ideal(spec, steps, threads):
+    from oracle import pyoracle as po
+
+    o = po.OSystem.from_spec(spec)
+    t0 = time.perf_counter()
+    o.ideal_run_nve(spec.dt, steps, spec.skin, threads)
+    return spec.n * steps / (time.perf_counter() - t0)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle force check")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU design is described in DESIGN.md, nothing here launches it).",
+              file=sys.stderr)
+        return 2
+
+    spec, desc = workload(args.config)
+    nbar = list_mean(spec)
+    log(f"{args.config}: N={spec.n}, mean list length {nbar:.1f}")
+
+    log(f"mixed precision: warmup {args.warmup}, timed {args.steps} steps")
+    mixed = run_gpu(spec, "mixed", args.steps, args.warmup)
+    log(f"mixed: {mixed['rate']:.4e} atom-steps/s, {mixed['ms_per_step']:.4f} ms/step, "
+        f"force kernel {mixed['force_ms']:.4f} ms, rebuilds {mixed['rebuilds']}")
+    fp64 = None
+    if not args.no_fp64:
+        log(f"fp64: warmup {args.warmup}, timed {args.steps} steps")
+        fp64 = run_gpu(spec, "fp64", args.steps, args.warmup)
+        log(f"fp64: {fp64['rate']:.4e} atom-steps/s, {fp64['ms_per_step']:.4f} ms/step, "
+            f"force kernel {fp64['force_ms']:.4f} ms")
+
+    fb = force_kernel_bytes_per_atom(nbar)
+    achieved = fb * spec.n / (mixed["force_ms"] * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "kernel": "k_force (mixed)",
+        "bytes_per_atom": round(fb, 1),
+        "force_kernel_ms": round(mixed["force_ms"], 5),
+    }
+
+    out_extra = {}
+    cpu = None
+    if not args.no_cpu:
+        log(f"CPU oracle: initial forces + {args.cpu_steps} faithful steps on 1 core")
+        f_ref, cpu_rate, t_force, t_steps = cpu_baseline(spec, args.cpu_steps)
+        rms = math.sqrt(float((f_ref ** 2).sum(axis=1).mean()))
+        out_extra["force_rms_err_mixed"] = float(
+            math.sqrt(float(((mixed["f0"] - f_ref) ** 2).sum(axis=1).mean())) / rms)
+        if fp64 is not None:
+            out_extra["force_max_err_fp64"] = float(np.abs(fp64["f0"] - f_ref).max() / rms)
+        cpu = {
+            "value": round(cpu_rate, 1),
+            "unit": "atom-steps/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": (f"{args.cpu_steps} NVE steps of the same {spec.n}-atom system with the oracle's "
+                       f"LJVOffsetManager + verlet_step_nve restatement ({t_steps:.1f} s), "
+                       f"{platform.processor() or platform.machine()}"),
+        }
+        log(f"CPU oracle: {cpu_rate:.4e} atom-steps/s (1 core)")
+
+    nsteps = args.steps
+    res = {
+        "metric": METRIC,
+        "value": round(mixed["rate"], 1),
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": nsteps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 pair arithmetic, fp64 state/integration/reductions",
+        "data": "synthetic (seeded fcc + jitter, Maxwell-Boltzmann velocities)",
+        "config": {
+            "workload": desc,
+            "n_atoms": spec.n,
+            "mean_list_length": round(nbar, 2),
+            "precision": "mixed",
+        },
+        "fp64_value": round(fp64["rate"], 1) if fp64 else None,
+        "fp64_ms_per_step": round(fp64["ms_per_step"], 5) if fp64 else None,
+        "mixed_over_fp64": round(mixed["rate"] / fp64["rate"], 3) if fp64 else None,
+        "nve_drift_per_atom_per_1e4_steps": {
+            "mixed": drift_per_atom_per_1e4(mixed["thermo"][:, 2], spec.n),
+            "fp64": drift_per_atom_per_1e4(fp64["thermo"][:, 2], spec.n) if fp64 else None,
+        },
+        "rebuilds": {"mixed": mixed["rebuilds"], "fp64": fp64["rebuilds"] if fp64 else None},
+        "step_bytes_per_atom": round(step_bytes_per_atom(nbar), 1),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "job_latency": None,
+    }
+    res.update(out_extra)
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

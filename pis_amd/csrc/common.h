@@ -20,6 +20,7 @@ constexpr int kTileThreads = 256;  // workgroup of the tile kernels (4 wave64)
 constexpr int kAtomThreads = 256;  // workgroup of the per-atom kernels
 constexpr int kMaxHalo = 256;      // halo-box cells a tile may stage
 constexpr int kMaxTypes = 4;
+constexpr int kMaxList = 512;      // neighbour-list capacity limit per atom
 
 // ctl[] slots (device ints per system)
 enum {
@@ -39,6 +40,10 @@ enum { MODE_TILE_REL = 0, MODE_RAW_MI = 1, MODE_WHOLE = 2 };
 struct PairParam {
     double r2_force;  // largest r^2 with sqrt_rn(r^2) <= rcut (-1: pair absent)
     double r2_list;   // largest r^2 with sqrt_rn(r^2) <= rcut + skin (-1: absent)
+    // Rounding band around each threshold: an r^2 computed from staged coordinates is
+    // within band of the reference's r^2, so below *_lo is surely inside, above *_hi
+    // surely outside, and only the band needs the bit-exact reference predicate.
+    double f_lo, f_hi, l_lo, l_hi;
     double sig2;      // sigma^2
     double c4;        // 4 eps
     double c24;       // 24 eps

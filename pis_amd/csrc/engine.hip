@@ -75,8 +75,8 @@ struct Plan {
 
 int scap_for(double need)
 {
-    const int opts[4] = {1024, 1536, 2048, 4096};
-    for (int k = 0; k < 4; ++k)
+    const int opts[7] = {1024, 1280, 1536, 1792, 2048, 3072, 4096};
+    for (int k = 0; k < 7; ++k)
         if (need <= opts[k]) return opts[k];
     return -1;
 }
@@ -177,6 +177,7 @@ pis_status make_plan(const pis_system_desc* d, Plan* p)
     for (auto& q : p->pp) {
         q.r2_force = -1.0;
         q.r2_list = -1.0;
+        q.f_lo = q.f_hi = q.l_lo = q.l_hi = -1.0;
     }
     p->rcmax = 0.0;
     for (uint32_t k = 0; k < d->n_pairs; ++k) {
@@ -212,6 +213,31 @@ pis_status make_plan(const pis_system_desc* d, Plan* p)
     }
     p->rcl = p->rcmax + d->skin;
     p->density = (double)p->n / p->volume;
+    // Rounding band of r^2 computed from staged (tile-relative or raw) coordinates versus
+    // the reference's r^2: the displacement differs by at most ~4 ulp(L) in each component
+    // plus a few relative ulps, so relative to r^2 at distance r the gap is below
+    // 8 L 2^-52 / r + 11 * 2^-53.  The band is 16 times that.
+    {
+        const double Lmax = std::max(p->L[0], std::max(p->L[1], p->L[2]));
+        const double u = std::ldexp(1.0, -52);
+        auto band = [&](double r) { return 16.0 * (8.0 * Lmax * u / r + 5.5 * u); };
+        for (uint32_t k = 0; k < d->n_pairs; ++k) {
+            const pis_pair_coeff& c = d->pairs[k];
+            const int a = (int)std::min(c.type_i, c.type_j) - 1, b = (int)std::max(c.type_i, c.type_j) - 1;
+            for (int q : {a * nt + b, b * nt + a}) {
+                PairParam& pp = p->pp[(size_t)q];
+                const double bf = band(c.rcut), bl = band(c.rcut + d->skin);
+                pp.f_lo = pp.r2_force * (1.0 - bf);
+                pp.f_hi = pp.r2_force * (1.0 + bf);
+                pp.l_lo = pp.r2_list * (1.0 - bl);
+                pp.l_hi = pp.r2_list * (1.0 + bl);
+            }
+        }
+        for (auto& q : p->pp)
+            if (q.r2_force < 0.0) {
+                q.f_lo = q.f_hi = q.l_lo = q.l_hi = -1.0;
+            }
+    }
     // Cell grid: edge strictly larger than rcut + skin (relative margin far above
     // rounding) so every pair within the list cutoff is in adjacent cells.
     bool thin = false;
@@ -254,7 +280,11 @@ pis_status make_plan(const pis_system_desc* d, Plan* p)
         maxnb = (int)d->neighbor_capacity;
     else
         maxnb = (int)std::ceil(1.6 * p->density * vsphere) + 48;
-    maxnb = std::min(maxnb, 65528);
+    if (maxnb > kMaxList) {
+        if (d->neighbor_capacity)
+            return fail(PIS_ERR_UNSUPPORTED, "neighbor_capacity above " + std::to_string(kMaxList));
+        maxnb = kMaxList;
+    }
     if (p->mode == MODE_WHOLE) maxnb = std::min(maxnb, (int)p->n + 8);
     p->maxnb = std::max(8, (maxnb + 7) / 8 * 8);
     return PIS_OK;
@@ -294,6 +324,7 @@ struct pis_ctx {
     uint64_t steps = 0, force_launches = 0;
     bool timing = false;
     std::vector<hipEvent_t> ev;
+    size_t ev_used = 0;
     double force_ms = 0.0;
     uint64_t timed_launches = 0;
 };
@@ -603,19 +634,22 @@ static pis_status ensure_thermo(pis_ctx* c, int cap)
     return PIS_OK;
 }
 
+// Force launches bracketed by HIP events on the context's stream while timing is on; the
+// events come from a pool that is reused between calls.
 static pis_status launch_force_timed(pis_ctx* c, int integrate)
 {
-    hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing) {
-        PIS_HIP(hipEventCreate(&e0));
-        PIS_HIP(hipEventCreate(&e1));
-        PIS_HIP(hipEventRecord(e0, c->stream));
+        while (c->ev.size() < c->ev_used + 2) {
+            hipEvent_t e = nullptr;
+            PIS_HIP(hipEventCreate(&e));
+            c->ev.push_back(e);
+        }
+        PIS_HIP(hipEventRecord(c->ev[c->ev_used], c->stream));
     }
     launch_force(c->stream, c->d_sys, c->d_tile_work, c->n_tile_work, c->scap, c->multi, c->mixed, integrate);
     if (c->timing) {
-        PIS_HIP(hipEventRecord(e1, c->stream));
-        c->ev.push_back(e0);
-        c->ev.push_back(e1);
+        PIS_HIP(hipEventRecord(c->ev[c->ev_used + 1], c->stream));
+        c->ev_used += 2;
     }
     c->force_launches++;
     return PIS_OK;
@@ -623,14 +657,13 @@ static pis_status launch_force_timed(pis_ctx* c, int integrate)
 
 static pis_status collect_timing(pis_ctx* c)
 {
-    for (size_t k = 0; k + 1 < c->ev.size(); k += 2) {
+    for (size_t k = 0; k + 1 < c->ev_used; k += 2) {
         float ms = 0.0f;
         PIS_HIP(hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]));
         c->force_ms += ms;
         c->timed_launches++;
     }
-    for (auto e : c->ev) (void)hipEventDestroy(e);
-    c->ev.clear();
+    c->ev_used = 0;
     return PIS_OK;
 }
 

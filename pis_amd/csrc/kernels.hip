@@ -21,6 +21,22 @@
 
 #include "kernels.h"
 
+// Tuning knobs fixed at build time: candidates of the force kernel whose LDS loads are
+// issued together (4 or 8); the minimum waves per SIMD its register allocation must allow
+// (launch-bounds occupancy target) per precision; the persistent grid of the list build.
+#ifndef PIS_FORCE_BATCH
+#define PIS_FORCE_BATCH 8
+#endif
+#ifndef PIS_FORCE_MIN_WAVES_MIXED
+#define PIS_FORCE_MIN_WAVES_MIXED 3
+#endif
+#ifndef PIS_FORCE_MIN_WAVES_FP64
+#define PIS_FORCE_MIN_WAVES_FP64 1
+#endif
+#ifndef PIS_BUILD_GRID_CAP
+#define PIS_BUILD_GRID_CAP (1 << 30)
+#endif
+
 namespace pis {
 
 // ---------------------------------------------------------------------------------
@@ -77,8 +93,8 @@ __device__ int block_excl_scan(int v, int* tmp, int* total)
     return add + x - v;
 }
 
-// Deterministic sum of 4 doubles over a 256-thread block; result valid in thread 0.
-__device__ void block_sum4(double v[4], double* tmp /* [4 waves * 4] */)
+// Deterministic sum of 4 doubles over the block (<= 16 waves); result valid in thread 0.
+__device__ void block_sum4(double v[4], double* tmp /* [16 waves * 4] */)
 {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
@@ -124,8 +140,8 @@ __device__ __forceinline__ int wrap_cell(int c, int n) { return c < 0 ? c + n : 
 enum { STAGE_RAW = 0, STAGE_REL = 1, STAGE_INDEX = 2 };
 
 // Stage the atoms of a tile's halo box (or of the whole system) into LDS, cell by cell
-// in halo order.  pref[e] is the first slot of halo cell e.  Slot SCAP is a NaN
-// sentinel (list padding points there; NaN fails every cutoff comparison).
+// in halo order.  pref[e] is the first slot of halo cell e.  Slot SCAP holds finite
+// padding (list entries past an atom's count point there and are masked by count).
 // Returns the number of staged atoms (uniform over the block); > SCAP means overflow
 // and nothing was staged.
 template <int SCAP, int WHAT>
@@ -184,9 +200,9 @@ __device__ int stage_tile(const SysDev& s, const State& st, const TileGeom& g, d
         }
     }
     if (tid == 0 && WHAT != STAGE_INDEX) {
-        Px[SCAP] = __longlong_as_double(0x7ff8000000000000ll);
-        Py[SCAP] = Px[SCAP];
-        Pz[SCAP] = Px[SCAP];
+        Px[SCAP] = 0.0;
+        Py[SCAP] = 0.0;
+        Pz[SCAP] = 0.0;
         Pt[SCAP] = 0;
     }
     __syncthreads();
@@ -332,26 +348,33 @@ __global__ __launch_bounds__(kAtomThreads) void k_bin(const SysDev* __restrict__
     }
 }
 
+// Exclusive scan of the cell histogram of one system by one workgroup, in coalesced
+// chunks of 4096 cells (4 per thread); also clears the histogram for the next build.
 __global__ __launch_bounds__(1024) void k_scan(const SysDev* __restrict__ sys)
 {
     __shared__ int tmp[16];
     const SysDev& s = sys[blockIdx.x];
     if (!s.ctl[CTL_REBUILD]) return;
     const int n = s.ncell;
-    const int per = (n + (int)blockDim.x - 1) / (int)blockDim.x;
-    const int b = min(n, (int)threadIdx.x * per), e = min(n, b + per);
-    int sum = 0;
-    for (int i = b; i < e; ++i) sum += s.cell_count[i];
-    int total;
-    int run = block_excl_scan(sum, tmp, &total);
-    for (int i = b; i < e; ++i) {
-        const int c = s.cell_count[i];
-        s.cell_start[i] = run;
-        run += c;
-        s.cell_count[i] = 0;
-        s.cell_fill[i] = 0;
+    int carry = 0;
+    for (int base = 0; base < n; base += 4 * (int)blockDim.x) {
+        const int i0 = base + 4 * (int)threadIdx.x;
+        int c[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) c[k] = (i0 + k < n) ? s.cell_count[i0 + k] : 0;
+        int total;
+        int run = carry + block_excl_scan(c[0] + c[1] + c[2] + c[3], tmp, &total);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (i0 + k < n) {
+                s.cell_start[i0 + k] = run;
+                run += c[k];
+                s.cell_count[i0 + k] = 0;
+                s.cell_fill[i0 + k] = 0;
+            }
+        carry += total;
     }
-    if (threadIdx.x == 0) s.cell_start[n] = total;
+    if (threadIdx.x == 0) s.cell_start[n] = carry;
 }
 
 __global__ __launch_bounds__(kAtomThreads) void k_scatter(const SysDev* __restrict__ sys,
@@ -410,11 +433,184 @@ __global__ __launch_bounds__(kAtomThreads) void k_rank_permute(const SysDev* __r
     }
 }
 
-// Full neighbour list at rcut_ij + skin.  Predicate exactly as build_neighbour_list
-// (lennard_jones.rs:398-405): rij = r_j - r_i, minimum image with round(), and
-// `norm() > rcut` rejects -- evaluated as r2 > r2_list where r2_list is the largest
-// double whose correctly rounded sqrt is <= rcut + skin, and r2 = (x^2+y^2)+z^2 rounded
-// like nalgebra's norm_squared.  Entries are LDS byte offsets of the tile's staging.
+// Atom index of LDS slot `sl` (the staging is cell by cell in halo order).
+__device__ __forceinline__ int slot_atom(const int* pref, const int* ecell, int ne, int sl)
+{
+    int lo = 0, hi = ne - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pref[mid] <= sl)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return ecell[lo] + (sl - pref[lo]);
+}
+
+// The reference's predicate for one pair, bit for bit: rij = r_j - r_i from the raw
+// positions, apply_boundary_conditions_dis with round(), r2 = (x^2+y^2)+z^2 without
+// fusion, accepted when r2 <= r2max (<=> norm() <= rcut).
+__device__ __forceinline__ bool exact_within(const SysDev& s, const State& st, int a, double xi,
+                                             double yi, double zi, double r2max)
+{
+    const double dx = mi_ref(st.x[a] - xi, s.L[0], s.Linv[0]);
+    const double dy = mi_ref(st.y[a] - yi, s.L[1], s.Linv[1]);
+    const double dz = mi_ref(st.z[a] - zi, s.L[2], s.Linv[2]);
+    return (dx * dx + dy * dy) + dz * dz <= r2max;
+}
+
+// Full neighbour list at rcut_ij + skin with the predicate of build_neighbour_list
+// (lennard_jones.rs:398-405) decided exactly.
+//
+// A wavefront owns one cell of the tile at a time and walks its atoms one by one.  For
+// each atom the 64 lanes sweep the cell's stencil -- the 9 contiguous (dy,dz) rows of the
+// staging -- 64 consecutive slots at a time (conflict-free LDS reads), so control flow is
+// uniform and the cutoff compare is directly the ballot of accepted candidates.  Accepted
+// lanes compact their entries into a per-wave LDS list at count + (accepted lanes below),
+// and the finished list leaves in whole uint4 groups, one store instruction per atom.
+// r2 from the staged coordinates decides every pair outside a +-band around the cutoff;
+// inside the band the bit-exact reference predicate (exact_within, raw positions) decides.
+// Entries are LDS byte offsets.
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Neighbour list of the own atom held by this lane (lane per atom).  The lane sweeps its
+// cell's stencil -- 9 contiguous (dy,dz) rows of the staging -- in chunks of up to 64
+// candidates with a trip count that is uniform over the wave (the wave's longest chunk),
+// setting one bit per accepted candidate; the set bits are then appended to a 128-bit
+// shift register that leaves as one uint4 list group per 8 entries.  r2 from the staged
+// coordinates decides every pair outside a +-band around the cutoff; inside the band the
+// bit-exact reference predicate (exact_within, raw positions) decides.
+template <int SCAP, bool PAIRMI, bool MULTI>
+__device__ __forceinline__ int build_atom(const SysDev& s, const State& st, const TileGeom& g,
+                                          const double* Px, const double* Py, const double* Pz,
+                                          const uint8_t* Pt, const int* pref, const int* ecell,
+                                          const PairParam* ppl, int total, bool active, int a,
+                                          double cx, double cy, double cz)
+{
+    const int nt = s.ntypes;
+    double xi = 0.0, yi = 0.0, zi = 0.0;
+    int ti = 0, e0 = 0, self = -1;
+    if (active) {
+        xi = st.x[a];
+        yi = st.y[a];
+        zi = st.z[a];
+        ti = st.type[a];
+        if (s.mode == MODE_WHOLE)
+            self = a;
+        else
+            own_slot(s, g, pref, a, &e0, &self);
+    }
+    double qx = xi, qy = yi, qz = zi;
+    if (!PAIRMI) {
+        qx = mi_fast(xi - cx, s.L[0], s.Linv[0]);
+        qy = mi_fast(yi - cy, s.L[1], s.Linv[1]);
+        qz = mi_fast(zi - cz, s.L[2], s.Linv[2]);
+    }
+    const double lo1 = ppl[0].l_lo, hi1 = ppl[0].l_hi, ex1 = ppl[0].r2_list;
+    uint64_t lo = 0, hi = 0;
+    int cnt = 0;
+    const int maxg = s.maxnb >> 3;
+    uint4* out = s.nbr + a;
+    const int nrows = s.mode == MODE_WHOLE ? 1 : 9;
+    const int hx = g.sx + 2, hxy = (g.sx + 2) * (g.sy + 2);
+    for (int r = 0; r < nrows; ++r) {
+        int rb = 0, re = 0;
+        if (active) {
+            if (s.mode == MODE_WHOLE) {
+                re = total;
+            } else {
+                const int e = e0 + ((r % 3) - 1) * hx + ((r / 3) - 1) * hxy;
+                rb = pref[e - 1];
+                re = pref[e + 2];
+            }
+        }
+        // uniform over the wave by construction; readfirstlane makes the loops scalar
+        const int rowmax = __builtin_amdgcn_readfirstlane(wave_max(re - rb));
+        for (int cb = 0; cb < rowmax; cb += 64) {
+            const int kmax = min(64, rowmax - cb);
+            uint32_t mlo = 0, mhi = 0;
+            bool band = false;
+            for (int k = 0; k < kmax; ++k) {
+                const int sl = rb + cb + k;
+                const bool valid = sl < re && sl != self;
+                const int sli = valid ? sl : SCAP;
+                double dx = Px[sli] - qx, dy = Py[sli] - qy, dz = Pz[sli] - qz;
+                if (PAIRMI) {
+                    dx = mi_fast(dx, s.L[0], s.Linv[0]);
+                    dy = mi_fast(dy, s.L[1], s.Linv[1]);
+                    dz = mi_fast(dz, s.L[2], s.Linv[2]);
+                }
+                const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                double blo = lo1, bhi = hi1;
+                if (MULTI) {
+                    const PairParam& p = ppl[ti * nt + Pt[sli]];
+                    blo = p.l_lo;
+                    bhi = p.l_hi;
+                }
+                const bool in = valid && r2 <= blo;
+                band |= valid && !in && r2 <= bhi;
+                const uint32_t bit = in ? 1u : 0u;
+                if (k < 32)
+                    mlo |= bit << k;
+                else
+                    mhi |= bit << (k - 32);
+            }
+            if (__any(band)) {
+                // rare: settle the band candidates with the reference predicate
+                for (int k = 0; k < kmax && band; ++k) {
+                    const int sl = rb + cb + k;
+                    if (sl >= re || sl == self) continue;
+                    double dx = Px[sl] - qx, dy = Py[sl] - qy, dz = Pz[sl] - qz;
+                    if (PAIRMI) {
+                        dx = mi_fast(dx, s.L[0], s.Linv[0]);
+                        dy = mi_fast(dy, s.L[1], s.Linv[1]);
+                        dz = mi_fast(dz, s.L[2], s.Linv[2]);
+                    }
+                    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                    const PairParam& p = ppl[MULTI ? ti * nt + Pt[sl] : 0];
+                    if (r2 > p.l_lo && r2 <= p.l_hi &&
+                        exact_within(s, st, slot_atom(pref, ecell, g.ne, sl), xi, yi, zi, p.r2_list)) {
+                        if (k < 32)
+                            mlo |= 1u << k;
+                        else
+                            mhi |= 1u << (k - 32);
+                    }
+                }
+            }
+            uint64_t mask = ((uint64_t)mhi << 32) | mlo;
+            while (mask) {
+                const int b = __builtin_ctzll(mask);
+                mask &= mask - 1;
+                lo = (lo >> 16) | (hi << 48);
+                hi = (hi >> 16) | ((uint64_t)((uint32_t)(rb + cb + b) << 3) << 48);
+                ++cnt;
+                if ((cnt & 7) == 0 && (cnt >> 3) <= maxg)
+                    out[(size_t)((cnt >> 3) - 1) * s.npad] =
+                        make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+            }
+        }
+    }
+    if (!active) return 0;
+    const int n_real = cnt;
+    if (cnt & 7) {
+        while (cnt & 7) {
+            lo = (lo >> 16) | (hi << 48);
+            hi = (hi >> 16) | ((uint64_t)(SCAP << 3) << 48);
+            ++cnt;
+        }
+        if ((cnt >> 3) <= maxg)
+            out[(size_t)((cnt >> 3) - 1) * s.npad] =
+                make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+    }
+    s.nnb[a] = (uint16_t)min(n_real, s.maxnb);
+    return n_real;
+}
+
 template <int SCAP>
 __global__ __launch_bounds__(kTileThreads) void k_build_list(const SysDev* __restrict__ sys,
                                                               const WorkItem* __restrict__ tiles,
@@ -425,7 +621,8 @@ __global__ __launch_bounds__(kTileThreads) void k_build_list(const SysDev* __res
     __shared__ int pref[kMaxHalo + 1];
     __shared__ int ecell[kMaxHalo];
     __shared__ int stmp[16];
-    __shared__ double r2l[kMaxTypes * kMaxTypes];
+    __shared__ PairParam ppl[kMaxTypes * kMaxTypes];
+    const int lane = threadIdx.x & 63;
     for (int b = blockIdx.x; b < ntw; b += gridDim.x) {
         const WorkItem w = tiles[xcd_remap(b, ntw)];
         const SysDev& s = sys[w.sys];
@@ -433,72 +630,39 @@ __global__ __launch_bounds__(kTileThreads) void k_build_list(const SysDev* __res
         const State st = s.st[s.ctl[CTL_CUR] ^ 1];
         const TileGeom g = tile_geom(s, w.idx);
         const int nt = s.ntypes;
-        for (int q = threadIdx.x; q < nt * nt; q += blockDim.x) r2l[q] = s.pp[q].r2_list;
-        const int total =
-            stage_tile<SCAP, STAGE_RAW>(s, st, g, 0.0, 0.0, 0.0, Px, Py, Pz, Pt, pref, ecell, stmp);
+        const bool rel = s.mode == MODE_TILE_REL;
+        const double cx = (g.x0 + 0.5 * g.sx) * s.cellw[0];
+        const double cy = (g.y0 + 0.5 * g.sy) * s.cellw[1];
+        const double cz = (g.z0 + 0.5 * g.sz) * s.cellw[2];
+        for (int q = threadIdx.x; q < nt * nt; q += blockDim.x) ppl[q] = s.pp[q];
+        int total;
+        if (rel)
+            total = stage_tile<SCAP, STAGE_REL>(s, st, g, cx, cy, cz, Px, Py, Pz, Pt, pref, ecell, stmp);
+        else
+            total = stage_tile<SCAP, STAGE_RAW>(s, st, g, cx, cy, cz, Px, Py, Pz, Pt, pref, ecell, stmp);
         if (total > SCAP) {
             if (threadIdx.x == 0) atomicOr(&s.ctl[CTL_OVF_STAGE], 1);
             __syncthreads();
             continue;
         }
         const int a0 = s.cell_start[g.c0], a1 = s.cell_start[g.c1];
-        const int hx = g.sx + 2, hxy = (g.sx + 2) * (g.sy + 2);
         int mymax = 0;
-        for (int a = a0 + (int)threadIdx.x; a < a1; a += blockDim.x) {
-            const double xi = st.x[a], yi = st.y[a], zi = st.z[a];
-            const int ti = st.type[a];
-            int e0 = 0, self;
-            if (s.mode == MODE_WHOLE)
-                self = a;
+        // whole waves iterate together (inactive lanes ride along) so the sweeps stay uniform
+        const int span = ((a1 - a0) + 63) & ~63;
+        for (int base = a0 + (int)(threadIdx.x & ~63); base < a0 + span; base += blockDim.x) {
+            const int a = base + lane;
+            const bool active = a < a1;
+            int n;
+            if (nt == 1)
+                n = rel ? build_atom<SCAP, false, false>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz)
+                        : build_atom<SCAP, true, false>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz);
             else
-                own_slot(s, g, pref, a, &e0, &self);
-            uint64_t lo = 0, hi = 0;
-            int cnt = 0;
-            const int maxg = s.maxnb >> 3;
-            uint4* out = s.nbr + a;
-            auto push = [&](uint32_t off) {
-                lo = (lo >> 16) | (hi << 48);
-                hi = (hi >> 16) | ((uint64_t)off << 48);
-                ++cnt;
-                if ((cnt & 7) == 0 && (cnt >> 3) <= maxg)
-                    out[(size_t)((cnt >> 3) - 1) * s.npad] =
-                        make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
-            };
-            auto test = [&](int sl) {
-                if (sl == self) return;
-                const double dx = mi_ref(Px[sl] - xi, s.L[0], s.Linv[0]);
-                const double dy = mi_ref(Py[sl] - yi, s.L[1], s.Linv[1]);
-                const double dz = mi_ref(Pz[sl] - zi, s.L[2], s.Linv[2]);
-                const double r2 = (dx * dx + dy * dy) + dz * dz;
-                if (r2 <= r2l[ti * nt + Pt[sl]]) push((uint32_t)sl << 3);
-            };
-            if (s.mode == MODE_WHOLE) {
-                for (int sl = 0; sl < total; ++sl) test(sl);
-            } else {
-                for (int dz = -1; dz <= 1; ++dz)
-                    for (int dy = -1; dy <= 1; ++dy) {
-                        const int e = e0 + dy * hx + dz * hxy;
-                        const int sb = pref[e - 1], se = pref[e + 2];
-                        for (int sl = sb; sl < se; ++sl) test(sl);
-                    }
-            }
-            const int n_real = cnt;
-            if (cnt & 7) {
-                while (cnt & 7) {
-                    lo = (lo >> 16) | (hi << 48);
-                    hi = (hi >> 16) | ((uint64_t)(SCAP << 3) << 48);
-                    ++cnt;
-                }
-                if ((cnt >> 3) <= maxg)
-                    out[(size_t)((cnt >> 3) - 1) * s.npad] =
-                        make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
-            }
-            s.nnb[a] = (uint16_t)min(n_real, s.maxnb);
-            mymax = max(mymax, n_real);
+                n = rel ? build_atom<SCAP, false, true>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz)
+                        : build_atom<SCAP, true, true>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz);
+            mymax = max(mymax, n);
         }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) mymax = max(mymax, __shfl_xor(mymax, o, 64));
-        if ((threadIdx.x & 63) == 0 && mymax > 0) {
+        mymax = wave_max(mymax);
+        if (lane == 0 && mymax > 0) {
             atomicMax(&s.ctl[CTL_MAXNB], mymax);
             if (mymax > s.maxnb) atomicOr(&s.ctl[CTL_OVF_NB], 1);
         }
@@ -513,80 +677,135 @@ struct Acc {
     double fx, fy, fz, pe, wp;
 };
 
-// One candidate.  FP64: LennardJones::compute_potential (lennard_jones.rs:32-55) with a
-// refined hardware reciprocal for 1/r2.  MIXED: the cutoff test and r2 stay FP64 (so the
-// pair set is the FP64 one), the LJ arithmetic is FP32, and the results are accumulated
-// in FP64.
+// LennardJones::compute_potential (lennard_jones.rs:32-55) for one candidate, without a
+// branch: `in` selects whether the pair contributes.  Operands are finite for every
+// candidate the list can hold (the padding slot is finite), so the unselected lanes only
+// cost time.  FP64: a refined hardware reciprocal for 1/r2.  MIXED: the LJ arithmetic is
+// FP32 (r2 and the cutoff decision stay FP64) and the results accumulate in FP64.
 template <bool MIXED>
-__device__ __forceinline__ void lj_eval(const PairParam& p, double dx, double dy, double dz, Acc& acc)
+__device__ __forceinline__ void lj_accum(const PairParam& p, double dx, double dy, double dz, double r2,
+                                         bool in, Acc& acc)
 {
-    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-    if (r2 <= p.r2_force) {
-        if (!MIXED) {
-            double inv = __builtin_amdgcn_rcp(r2);
-            inv = fma(inv, fma(-r2, inv, 1.0), inv);
-            inv = fma(inv, fma(-r2, inv, 1.0), inv);
-            const double sr2 = p.sig2 * inv;
-            const double sr6 = sr2 * sr2 * sr2;
-            const double fp = fma(p.c48, sr6, -p.c24) * sr6 * inv;
-            acc.fx = fma(-fp, dx, acc.fx);
-            acc.fy = fma(-fp, dy, acc.fy);
-            acc.fz = fma(-fp, dz, acc.fz);
-            acc.pe += fma(fma(p.c4, sr6, -p.c4), sr6, -p.ucut);
-            acc.wp = fma(fp, r2, acc.wp);
-        } else {
-            const float r2f = (float)r2;
-            const float inv = __builtin_amdgcn_rcpf(r2f);
-            const float sr2 = p.sig2f * inv;
-            const float sr6 = sr2 * sr2 * sr2;
-            const float fpf = fmaf(p.c48f, sr6, -p.c24f) * sr6 * inv;
-            const float uf = fmaf(fmaf(p.c4f, sr6, -p.c4f), sr6, -p.ucutf);
-            const double fp = (double)fpf;
-            acc.fx = fma(-fp, dx, acc.fx);
-            acc.fy = fma(-fp, dy, acc.fy);
-            acc.fz = fma(-fp, dz, acc.fz);
-            acc.pe += (double)uf;
-            acc.wp = fma(fp, r2, acc.wp);
-        }
+    if (!MIXED) {
+        double inv = __builtin_amdgcn_rcp(r2);
+        inv = fma(inv, fma(-r2, inv, 1.0), inv);
+        inv = fma(inv, fma(-r2, inv, 1.0), inv);
+        const double sr2 = p.sig2 * inv;
+        const double sr6 = sr2 * sr2 * sr2;
+        const double fp = in ? fma(p.c48, sr6, -p.c24) * sr6 * inv : 0.0;
+        const double u = in ? fma(fma(p.c4, sr6, -p.c4), sr6, -p.ucut) : 0.0;
+        acc.fx = fma(-fp, dx, acc.fx);
+        acc.fy = fma(-fp, dy, acc.fy);
+        acc.fz = fma(-fp, dz, acc.fz);
+        acc.pe += u;
+        acc.wp = fma(fp, r2, acc.wp);
+    } else {
+        const float r2f = (float)r2;
+        const float inv = __builtin_amdgcn_rcpf(r2f);
+        const float sr2 = p.sig2f * inv;
+        const float sr6 = sr2 * sr2 * sr2;
+        const float fpf = in ? fmaf(p.c48f, sr6, -p.c24f) * sr6 * inv : 0.0f;
+        const float uf = in ? fmaf(fmaf(p.c4f, sr6, -p.c4f), sr6, -p.ucutf) : 0.0f;
+        const double fp = (double)fpf;
+        acc.fx = fma(-fp, dx, acc.fx);
+        acc.fy = fma(-fp, dy, acc.fy);
+        acc.fz = fma(-fp, dz, acc.fz);
+        acc.pe += (double)uf;
+        acc.wp = fma(fp, r2, acc.wp);
     }
 }
+
+struct ExactCtx {
+    const SysDev* s;
+    const State* st;
+    const int* pref;
+    const int* ecell;
+    int ne;
+    double xi, yi, zi;
+};
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const u32x4 gu32x4;
 
 template <int SCAP, bool MIXED, bool MULTI, bool PAIRMI>
 __device__ __forceinline__ void atom_forces(const SysDev& s, const double* Px, const double* Py,
                                             const double* Pz, const uint8_t* Pt,
                                             const PairParam* ppl, const PairParam& p0, int a, int ti,
-                                            double qx, double qy, double qz, Acc& acc)
+                                            double qx, double qy, double qz, const ExactCtx& ex,
+                                            Acc& acc)
 {
     const int nn = s.nnb[a];
     const int ng = (nn + 7) >> 3;
-    const uint4* L = s.nbr + a;
+    const gu32x4* L = (const gu32x4*)(s.nbr + a);
     const char* bx = reinterpret_cast<const char*>(Px);
     const char* by = reinterpret_cast<const char*>(Py);
     const char* bz = reinterpret_cast<const char*>(Pz);
+    u32x4 un = {0u, 0u, 0u, 0u};
+    if (ng > 0) un = L[0];
     for (int gi = 0; gi < ng; ++gi) {
-        const uint4 u = L[(size_t)gi * s.npad];
+        const u32x4 u = un;
+        if (gi + 1 < ng) un = L[(size_t)(gi + 1) * s.npad];
         const uint32_t wv[4] = {u.x, u.y, u.z, u.w};
+        const int rem = nn - gi * 8;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const uint32_t off = (wv[e >> 1] >> ((e & 1) * 16)) & 0xffffu;
-            double dx = *reinterpret_cast<const double*>(bx + off) - qx;
-            double dy = *reinterpret_cast<const double*>(by + off) - qy;
-            double dz = *reinterpret_cast<const double*>(bz + off) - qz;
-            if (PAIRMI) {
-                dx = mi_fast(dx, s.L[0], s.Linv[0]);
-                dy = mi_fast(dy, s.L[1], s.Linv[1]);
-                dz = mi_fast(dz, s.L[2], s.Linv[2]);
+        for (int h = 0; h < 8; h += PIS_FORCE_BATCH) {
+            constexpr int B = PIS_FORCE_BATCH;
+            uint32_t off[B];
+            double X[B], Y[B], Z[B];
+#pragma unroll
+            for (int e = 0; e < B; ++e) {
+                const int k = h + e;
+                off[e] = (wv[k >> 1] >> ((k & 1) * 16)) & 0xffffu;
+                X[e] = *reinterpret_cast<const double*>(bx + off[e]);
+                Y[e] = *reinterpret_cast<const double*>(by + off[e]);
+                Z[e] = *reinterpret_cast<const double*>(bz + off[e]);
             }
-            if (MULTI)
-                lj_eval<MIXED>(ppl[ti * s.ntypes + Pt[off >> 3]], dx, dy, dz, acc);
-            else
-                lj_eval<MIXED>(p0, dx, dy, dz, acc);
+            double R2[B];
+            bool IN[B];
+            bool band = false;
+#pragma unroll
+            for (int e = 0; e < B; ++e) {
+                double dx = X[e] - qx, dy = Y[e] - qy, dz = Z[e] - qz;
+                if (PAIRMI) {
+                    dx = mi_fast(dx, s.L[0], s.Linv[0]);
+                    dy = mi_fast(dy, s.L[1], s.Linv[1]);
+                    dz = mi_fast(dz, s.L[2], s.Linv[2]);
+                }
+                X[e] = dx;
+                Y[e] = dy;
+                Z[e] = dz;
+                R2[e] = fma(dx, dx, fma(dy, dy, dz * dz));
+                const PairParam& p = MULTI ? ppl[ti * s.ntypes + Pt[off[e] >> 3]] : p0;
+                const bool valid = h + e < rem;
+                if (MIXED) {
+                    IN[e] = valid && R2[e] <= p.r2_force;
+                } else {
+                    IN[e] = valid && R2[e] <= p.f_lo;
+                    band |= valid && !IN[e] && R2[e] <= p.f_hi;
+                }
+            }
+            if (!MIXED && band) {
+                // the reference's `norm() > rcut` settled exactly inside the rounding band
+#pragma unroll
+                for (int e = 0; e < B; ++e) {
+                    const PairParam& p = MULTI ? ppl[ti * s.ntypes + Pt[off[e] >> 3]] : p0;
+                    if (h + e < rem && !IN[e] && R2[e] <= p.f_hi)
+                        IN[e] = exact_within(*ex.s, *ex.st,
+                                             slot_atom(ex.pref, ex.ecell, ex.ne, (int)(off[e] >> 3)), ex.xi,
+                                             ex.yi, ex.zi, p.r2_force);
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < B; ++e) {
+                const PairParam& p = MULTI ? ppl[ti * s.ntypes + Pt[off[e] >> 3]] : p0;
+                lj_accum<MIXED>(p, X[e], Y[e], Z[e], R2[e], IN[e], acc);
+            }
         }
     }
 }
 
 template <int SCAP, bool MIXED>
-__global__ __launch_bounds__(kTileThreads) void k_force(const SysDev* __restrict__ sys,
+__global__ __launch_bounds__(kTileThreads, MIXED ? PIS_FORCE_MIN_WAVES_MIXED : PIS_FORCE_MIN_WAVES_FP64) void k_force(const SysDev* __restrict__ sys,
                                                          const WorkItem* __restrict__ tiles, int ntw,
                                                          int integrate)
 {
@@ -595,7 +814,7 @@ __global__ __launch_bounds__(kTileThreads) void k_force(const SysDev* __restrict
     __shared__ int pref[kMaxHalo + 1];
     __shared__ int ecell[kMaxHalo];
     __shared__ int stmp[16];
-    __shared__ double rtmp[16];
+    __shared__ double rtmp[64];
     __shared__ PairParam ppl[kMaxTypes * kMaxTypes];
 
     const WorkItem w = tiles[xcd_remap(blockIdx.x, ntw)];
@@ -627,16 +846,17 @@ __global__ __launch_bounds__(kTileThreads) void k_force(const SysDev* __restrict
                 qz = mi_fast(zi - cz, s.L[2], s.Linv[2]);
             }
             Acc acc = {0.0, 0.0, 0.0, 0.0, 0.0};
+            const ExactCtx ex = {&s, &st, pref, ecell, g.ne, xi, yi, zi};
             if (nt == 1) {
                 if (rel)
-                    atom_forces<SCAP, MIXED, false, false>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, acc);
+                    atom_forces<SCAP, MIXED, false, false>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
                 else
-                    atom_forces<SCAP, MIXED, false, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, acc);
+                    atom_forces<SCAP, MIXED, false, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
             } else {
                 if (rel)
-                    atom_forces<SCAP, MIXED, true, false>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, acc);
+                    atom_forces<SCAP, MIXED, true, false>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
                 else
-                    atom_forces<SCAP, MIXED, true, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, acc);
+                    atom_forces<SCAP, MIXED, true, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
             }
             const double m = s.mass[ti];
             double vx = st.vx[a], vy = st.vy[a], vz = st.vz[a];
@@ -673,13 +893,19 @@ __global__ __launch_bounds__(kTileThreads) void k_force(const SysDev* __restrict
 
 // Per-system fixed-order sum of the tile partials -> thermo record; optionally flips the
 // live buffer set after a rebuild.
-__global__ __launch_bounds__(256) void k_reduce(const SysDev* __restrict__ sys, int flip, int record)
+__global__ __launch_bounds__(1024) void k_reduce(const SysDev* __restrict__ sys, int flip, int record)
 {
-    __shared__ double rtmp[16];
+    __shared__ double rtmp[64];
     const SysDev& s = sys[blockIdx.x];
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int t = threadIdx.x; t < s.ntile; t += blockDim.x)
-        for (int k = 0; k < 4; ++k) v[k] += s.partial[4 * (size_t)t + k];
+    const double4* P4 = reinterpret_cast<const double4*>(s.partial);
+    for (int t = threadIdx.x; t < s.ntile; t += blockDim.x) {
+        const double4 q = P4[t];
+        v[0] += q.x;
+        v[1] += q.y;
+        v[2] += q.z;
+        v[3] += q.w;
+    }
     block_sum4(v, rtmp);
     if (threadIdx.x != 0) return;
     const double pe = 0.5 * v[0], ke = v[1], wabs = v[2], wpair = 0.5 * v[3];
@@ -759,10 +985,16 @@ __global__ __launch_bounds__(kTileThreads) void k_list_to_orig(const SysDev* __r
     do {                            \
         if ((scap) <= 1024)         \
             M(1024);                \
+        else if ((scap) <= 1280)    \
+            M(1280);                \
         else if ((scap) <= 1536)    \
             M(1536);                \
+        else if ((scap) <= 1792)    \
+            M(1792);                \
         else if ((scap) <= 2048)    \
             M(2048);                \
+        else if ((scap) <= 3072)    \
+            M(3072);                \
         else                        \
             M(4096);                \
     } while (0)
@@ -801,13 +1033,15 @@ void launch_rebuild(hipStream_t st, const SysDev* d_sys, int nsys, const WorkIte
                     bool multi_type)
 {
     (void)multi_type;
-    const int ga = grid_cap(n_atom_work, 2048);
+    // Rebuild kernels loop over their work items, so a modest grid covers any size and
+    // costs little on the steps where they return at once.
+    const int ga = grid_cap(n_atom_work, 512);
     hipLaunchKernelGGL(k_bin, dim3(ga), dim3(kAtomThreads), 0, st, d_sys, atom_work, n_atom_work);
     hipLaunchKernelGGL(k_scan, dim3(nsys), dim3(1024), 0, st, d_sys);
     hipLaunchKernelGGL(k_scatter, dim3(ga), dim3(kAtomThreads), 0, st, d_sys, atom_work, n_atom_work);
     hipLaunchKernelGGL(k_rank_permute, dim3(ga), dim3(kAtomThreads), 0, st, d_sys, atom_work,
                        n_atom_work);
-    const int gt = grid_cap(n_tile_work, 2048);
+    const int gt = grid_cap(n_tile_work, PIS_BUILD_GRID_CAP);
 #define PIS_BUILD(S) \
     hipLaunchKernelGGL(k_build_list<S>, dim3(gt), dim3(kTileThreads), 0, st, d_sys, tile_work, n_tile_work)
     PIS_SCAP_DISPATCH(scap, PIS_BUILD);
@@ -833,7 +1067,7 @@ void launch_force(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work
 
 void launch_reduce(hipStream_t st, const SysDev* d_sys, int nsys, int flip, int record)
 {
-    hipLaunchKernelGGL(k_reduce, dim3(nsys), dim3(256), 0, st, d_sys, flip, record);
+    hipLaunchKernelGGL(k_reduce, dim3(nsys), dim3(1024), 0, st, d_sys, flip, record);
 }
 
 void launch_flip(hipStream_t st, const SysDev* d_sys, int nsys)
