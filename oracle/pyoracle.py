@@ -234,6 +234,12 @@ def min_image(box: Box, d):
     return d
 
 
+def orc_norm_py(v):
+    """nalgebra norm() as the oracle computes it (sqrt of (x^2+y^2)+z^2)."""
+    v = np.ascontiguousarray(np.array(v, dtype=np.float64))
+    return lib().orc_norm(v.ctypes.data_as(_D))
+
+
 def wrap(box: Box, r):
     r = np.ascontiguousarray(np.array(r, dtype=np.float64))
     lib().orc_wrap(C.byref(box), r.ctypes.data_as(_D))
