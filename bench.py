@@ -70,6 +70,27 @@ def step_bytes_per_atom(nbar):
     return force_kernel_bytes_per_atom(nbar) + (24 * 3 + 1 + 24 + 24)
 
 
+def measured_traffic(natoms):
+    """HBM bytes per force launch from the committed rocprofv3 counter summary
+    (FETCH_SIZE / WRITE_SIZE scaled by a streaming copy of known size), rescaled to the
+    atom count; None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_force_lj_liquid_1M_traffic.md")
+    if not os.path.exists(path):
+        return None
+    kv = {}
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 2 and parts[0].endswith(("_kib", "_dispatch", "_bytes", "atoms")):
+            kv[parts[0]] = float(parts[1])
+    need = ("fetch_kib_per_dispatch", "write_kib_per_dispatch", "copy_fetch_kib", "copy_write_kib", "copy_bytes", "atoms")
+    if not all(k in kv for k in need):
+        return None
+    copy_kib = kv["copy_bytes"] / 1024.0
+    b = 1024.0 * (kv["fetch_kib_per_dispatch"] * copy_kib / kv["copy_fetch_kib"]
+                  + kv["write_kib_per_dispatch"] * copy_kib / kv["copy_write_kib"])
+    return b * natoms / kv["atoms"]
+
+
 def drift_per_atom_per_1e4(etot, n):
     t = np.arange(etot.shape[0], dtype=np.float64)
     slope = np.polyfit(t, etot - etot[0], 1)[0]
@@ -122,6 +143,16 @@ def cpu_baseline(spec, steps):
     return f_ref, spec.n * steps / (t3 - t2), t1 - t0, t3 - t2
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
 def cpu_ideal(spec, steps, threads):
     from oracle import pyoracle as po
 
@@ -140,6 +171,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle force check")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30,
+                    help="bytes of the streaming-copy calibration run after the timed runs (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -162,6 +195,17 @@ def main():
         log(f"fp64: {fp64['rate']:.4e} atom-steps/s, {fp64['ms_per_step']:.4f} ms/step, "
             f"force kernel {fp64['force_ms']:.4f} ms")
 
+    copy_gbs = None
+    if args.copy_bytes:
+        import ctypes as C
+
+        from pis_amd import _capi
+
+        ms = C.c_double(0.0)
+        _capi.check(_capi.lib().pis_copy_calibration(args.copy_bytes, 5, C.byref(ms)))
+        copy_gbs = 2.0 * args.copy_bytes / (ms.value * 1e-3) / 1e9
+        log(f"streaming copy of {args.copy_bytes} B: {copy_gbs:.0f} GB/s (read+write)")
+
     fb = force_kernel_bytes_per_atom(nbar)
     achieved = fb * spec.n / (mixed["force_ms"] * 1e-3) / 1e9
     roofline = {
@@ -170,10 +214,12 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": (round(measured_traffic(spec.n)) if measured_traffic(spec.n) and args.config == "c2" else None),
+        "traffic_source": "profiles/k_force_lj_liquid_1M_traffic.md (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-kernel scaled)",
         "kernel": "k_force (mixed)",
         "bytes_per_atom": round(fb, 1),
         "force_kernel_ms": round(mixed["force_ms"], 5),
+        "copy_kernel_GBs": round(copy_gbs, 1) if copy_gbs else None,
     }
 
     out_extra = {}
@@ -192,10 +238,21 @@ def main():
             "cores": 1,
             "kind": "port",
             "sample": (f"{args.cpu_steps} NVE steps of the same {spec.n}-atom system with the oracle's "
-                       f"LJVOffsetManager + verlet_step_nve restatement ({t_steps:.1f} s), "
-                       f"{platform.processor() or platform.machine()}"),
+                       f"LJVOffsetManager + verlet_step_nve restatement ({t_steps:.1f} s), {cpu_model()}"),
         }
         log(f"CPU oracle: {cpu_rate:.4e} atom-steps/s (1 core)")
+        ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        ncores = min(ncores or 1, 16)
+        ideal = cpu_ideal(spec, args.cpu_steps, ncores)
+        out_extra["cpu_ideal_parallel"] = {
+            "value": round(ideal, 1),
+            "unit": "atom-steps/s",
+            "cores": ncores,
+            "kind": "idealised lock-free full-list OpenMP port (not the shipped rayon code)",
+            "sample": f"{args.cpu_steps} NVE steps incl. the initial list build",
+        }
+        out_extra["host_cpu"] = cpu_model()
+        log(f"CPU ideal parallel: {ideal:.4e} atom-steps/s on {ncores} threads")
 
     nsteps = args.steps
     res = {

@@ -168,6 +168,12 @@ pis_status pis_get_stats(pis_ctx* ctx, pis_stats* out);
 /* Turn per-launch HIP-event timing of the force kernel on (1) or off (0); resets sums. */
 pis_status pis_set_kernel_timing(pis_ctx* ctx, int32_t enable);
 
+/* Measurement utility (no reference counterpart): `reps` launches of a plain streaming copy
+ * kernel that reads and writes exactly `bytes` (rounded down to 32); *ms_per_copy gets the
+ * average HIP-event time.  Used to scale hardware byte counters and to measure the
+ * achievable HBM bandwidth on the running device. */
+pis_status pis_copy_calibration(uint64_t bytes, int32_t reps, double* ms_per_copy);
+
 /* ---- Batched ensembles: many independent systems advanced by one launch per kernel
  * per step (no reference counterpart beyond running PotentialManager per system). */
 pis_status pis_batch_create(const pis_system_desc* descs, uint32_t n_systems, pis_ctx** out);

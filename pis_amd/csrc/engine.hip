@@ -930,6 +930,40 @@ pis_status pis_get_stats(pis_ctx* ctx, pis_stats* out)
     return PIS_OK;
 }
 
+pis_status pis_copy_calibration(uint64_t bytes, int32_t reps, double* ms_per_copy)
+{
+    if (bytes < 32 || reps < 1) return fail(PIS_ERR_INVALID_ARGUMENT, "bytes >= 32 and reps >= 1 required");
+    bytes &= ~(uint64_t)31;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(PIS_ERR_NO_DEVICE, "no HIP device available");
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&a, bytes);
+    if (e == hipSuccess) e = hipMalloc(&b, bytes);
+    if (e == hipSuccess) e = hipMemset(a, 0, bytes);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) {
+        launch_copy_calibration(st, a, b, bytes);  // warm-up
+        e = hipEventRecord(e0, st);
+    }
+    for (int r = 0; e == hipSuccess && r < reps; ++r) launch_copy_calibration(st, a, b, bytes);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (ms_per_copy) *ms_per_copy = ms / reps;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e != hipSuccess) return fail(PIS_ERR_HIP, std::string("copy calibration: ") + hipGetErrorString(e));
+    return PIS_OK;
+}
+
 pis_status pis_set_kernel_timing(pis_ctx* ctx, int32_t enable)
 {
     if (!ctx) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");

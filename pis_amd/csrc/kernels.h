@@ -26,4 +26,6 @@ void launch_list_to_orig(hipStream_t st, const SysDev* d_sys, const WorkItem* ti
                          int n_tile_work, int scap, bool multi_type, uint32_t* out,
                          uint32_t* counts);
 
+void launch_copy_calibration(hipStream_t st, const void* src, void* dst, size_t bytes);
+
 }  // namespace pis

@@ -977,8 +977,23 @@ __global__ __launch_bounds__(kTileThreads) void k_list_to_orig(const SysDev* __r
     }
 }
 
+// Plain streaming copy used to calibrate hardware byte counters: reads and writes exactly
+// n doubles.
+__global__ __launch_bounds__(256) void k_copy_calibration(const double4* __restrict__ src,
+                                                          double4* __restrict__ dst, size_t n4)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------------
 // launch wrappers
+
+void launch_copy_calibration(hipStream_t st, const void* src, void* dst, size_t bytes)
+{
+    const size_t n4 = bytes / 32;
+    hipLaunchKernelGGL(k_copy_calibration, dim3(4096), dim3(256), 0, st, (const double4*)src, (double4*)dst, n4);
+}
 
 // Staging capacities with compiled kernel instances.
 #define PIS_SCAP_DISPATCH(scap, M) \
