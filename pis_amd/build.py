@@ -42,5 +42,22 @@ def build(verbose: bool = False, out: str = OUT, defines=()) -> str:
     return out
 
 
+def build_cpp_test() -> str:
+    """The C++ host-mirror parity test (tests/cpp/test_manager.cpp), host code only: g++
+    against libpis_hip.so and the oracle, with $ORIGIN-relative run paths."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_manager.cpp")
+    outdir = os.path.join(ROOT, "tests", "cpp", "build")
+    os.makedirs(outdir, exist_ok=True)
+    out = os.path.join(outdir, "test_manager")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-o", out, src,
+           "-L", HERE, "-lpis_hip", "-L", os.path.join(ROOT, "oracle"), "-l:liboracle.so",
+           "-Wl,-rpath,$ORIGIN/../../../pis_amd:$ORIGIN/../../../oracle"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed for tests/cpp/test_manager")
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
