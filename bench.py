@@ -125,6 +125,40 @@ def run_gpu(spec, precision, steps, warmup):
     }
 
 
+def run_ensemble(steps, warmup, direct_steps):
+    """Batched ensemble: 512 Kob-Andersen systems (256-32,768 atoms, log-uniform from seed 7,
+    density 1.2, T* 0.5-1.5) in one context, mixed precision.  Each step is one replay of
+    the captured step graph (one launch per kernel per LDS staging class over all systems);
+    the same batch with direct launches is timed beside it."""
+    from pis_amd import Engine, workloads as W
+
+    specs = W.ka_ensemble(512, seed=7, nmin=256, nmax=32768, steps=steps)
+    natoms = sum(s.n for s in specs)
+    dts = [s.dt for s in specs]
+    out = {"n_systems": len(specs), "total_atoms": natoms, "steps": steps,
+           "min_atoms": min(s.n for s in specs), "max_atoms": max(s.n for s in specs)}
+    with Engine(specs, "mixed") as eng:
+        eng.load()
+        eng.compute_potential()
+        eng.set_graph_mode(True)
+        eng.run_nve(dts, warmup, record=False)
+        t0 = time.perf_counter()
+        eng.run_nve(dts, steps, record=False)
+        sec = time.perf_counter() - t0
+        out["value"] = round(natoms * steps / sec, 1)
+        out["ms_per_step"] = round(1e3 * sec / steps, 5)
+        out["rebuilds_per_system"] = round(eng.stats()["rebuilds"] / len(specs), 1)
+        if direct_steps:
+            eng.set_graph_mode(False)
+            t0 = time.perf_counter()
+            eng.run_nve(dts, direct_steps, record=False)
+            sec = time.perf_counter() - t0
+            out["direct_launch_value"] = round(natoms * direct_steps / sec, 1)
+            out["direct_launch_steps"] = direct_steps
+    out["unit"] = "atom-steps/s"
+    return out
+
+
 def cpu_baseline(spec, steps):
     """The oracle (faithful LJVOffsetManager + verlet_step_nve, 1 core) on `steps` steps
     of the same system; also returns its initial forces for the force-error figures."""
@@ -171,6 +205,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle force check")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--ensemble-steps", type=int, default=5000,
+                    help="steps of the 512-system batched ensemble (0: skip)")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30,
                     help="bytes of the streaming-copy calibration run after the timed runs (0: skip)")
     args = ap.parse_args()
@@ -194,6 +230,13 @@ def main():
         fp64 = run_gpu(spec, "fp64", args.steps, args.warmup)
         log(f"fp64: {fp64['rate']:.4e} atom-steps/s, {fp64['ms_per_step']:.4f} ms/step, "
             f"force kernel {fp64['force_ms']:.4f} ms")
+
+    ensemble = None
+    if args.ensemble_steps:
+        log(f"batched ensemble: 512 KA systems, {args.ensemble_steps} steps, graph replay")
+        ensemble = run_ensemble(args.ensemble_steps, 100, min(args.ensemble_steps, 1000))
+        log(f"ensemble: {ensemble['total_atoms']} atoms, {ensemble['value']:.4e} atom-steps/s "
+            f"(direct launches {ensemble.get('direct_launch_value', 0):.4e})")
 
     copy_gbs = None
     if args.copy_bytes:
@@ -285,6 +328,7 @@ def main():
         "step_bytes_per_atom": round(step_bytes_per_atom(nbar), 1),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched_ensemble": ensemble,
         "job_latency": None,
     }
     res.update(out_extra)

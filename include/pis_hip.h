@@ -165,6 +165,9 @@ pis_status pis_get_neighbor_list(pis_ctx* ctx, uint64_t* offsets, uint32_t* idx,
                                  uint64_t* total);
 
 pis_status pis_get_stats(pis_ctx* ctx, pis_stats* out);
+/* Replay one captured hipGraph per step in pis_verlet_step_nve / pis_batch_verlet_step_nve
+ * (1) instead of launching each kernel (0, the default).  Ignored while kernel timing is on. */
+pis_status pis_set_graph_mode(pis_ctx* ctx, int32_t enable);
 /* Turn per-launch HIP-event timing of the force kernel on (1) or off (0); resets sums. */
 pis_status pis_set_kernel_timing(pis_ctx* ctx, int32_t enable);
 

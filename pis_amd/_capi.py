@@ -123,6 +123,7 @@ SIGNATURES = {
     ),
     "pis_get_stats": (C.c_int, [_P, C.POINTER(Stats)]),
     "pis_set_kernel_timing": (C.c_int, [_P, C.c_int32]),
+    "pis_set_graph_mode": (C.c_int, [_P, C.c_int32]),
     "pis_copy_calibration": (C.c_int, [C.c_uint64, C.c_int32, _DP]),
     "pis_batch_create": (C.c_int, [C.POINTER(SystemDesc), C.c_uint32, C.POINTER(_P)]),
     "pis_batch_size": (C.c_uint32, [_P]),

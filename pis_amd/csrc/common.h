@@ -32,7 +32,8 @@ enum {
     CTL_MAXNB = 5,     // largest list length at the last build
     CTL_NREBUILD = 6,  // builds performed
     CTL_STEP = 7,      // thermo record index
-    CTL_COUNT = 8
+    CTL_MAXSTAGE = 8,  // most atoms staged by one tile since the state was set
+    CTL_COUNT = 9
 };
 
 enum { MODE_TILE_REL = 0, MODE_RAW_MI = 1, MODE_WHOLE = 2 };
@@ -50,6 +51,9 @@ struct PairParam {
     double c48;       // 48 eps
     double ucut;      // energy shift U(rcut) (0 without shift)
     float sig2f, c4f, c24f, c48f, ucutf, pad;
+    // list-band edges for r^2 from the list build's FP32 staging (wider: the staged
+    // coordinates carry FP32 rounding); -1 for an absent pair
+    float l_lo32, l_hi32;
 };
 
 struct State {
@@ -85,6 +89,7 @@ struct SysDev {
     double L[3], Linv[3];  // h diagonal and h_inv diagonal (caller's bits)
     double cellw[3];
     double half_skin2;     // (skin/2)^2
+    double wslack;         // x-window slack of the list build: FP32 staging error + margin
     double dt, dt2;
     double kb, volume;
     int32_t n, npad, ntypes, maxnb;

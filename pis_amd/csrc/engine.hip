@@ -2,12 +2,13 @@
 // device-resident step loop and the C ABI of include/pis_hip.h.
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
 
-#include "../../include/pis_hip.h"
+#include "pis_hip.h"
 #include "common.h"
 #include "kernels.h"
 
@@ -66,7 +67,7 @@ struct Plan {
     int nc[3] = {1, 1, 1}, tile[3] = {1, 1, 1}, nt[3] = {1, 1, 1};
     int ncell = 1, ntile = 1, mode = MODE_WHOLE, maxnb = 8, scap = 1024, npad = 64;
     double L[3] = {0, 0, 0}, Linv[3] = {0, 0, 0}, cellw[3] = {0, 0, 0};
-    double rcmax = 0, rcl = 0, skin = 0, density = 0, volume = 0, kb = 1;
+    double rcmax = 0, rcl = 0, skin = 0, density = 0, volume = 0, kb = 1, wslack = 0;
     bool mixed = false;
     std::vector<PairParam> pp;
     std::vector<double> mass;
@@ -274,6 +275,36 @@ pis_status make_plan(const pis_system_desc* d, Plan* p)
         if (p->scap < 0) return fail(PIS_ERR_UNSUPPORTED, "tile staging exceeds LDS capacity");
     }
     p->ntile = p->nt[0] * p->nt[1] * p->nt[2];
+    // Band for r^2 from the list build's FP32 staging.  Staged coordinates lie within
+    // qmax of the origin (the halo half-extent for tile-relative staging, the box for raw
+    // positions) and round to FP32 with error 2^-24 qmax, so a displacement component is
+    // within e = 6 * 2^-24 * qmax of its exact value (difference and image shift rounded
+    // once more), and at distance r, r^2 is within 2 sqrt(3) e r + 3 e^2 + 3 * 2^-24 r^2.
+    // The band is 4 times that; the x windows widen by twice the coordinate error.
+    {
+        const double u32 = std::ldexp(1.0, -24);
+        double qmax = 0.0, cwmax = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            qmax = std::max(qmax, p->mode == MODE_TILE_REL ? 0.5 * (p->tile[k] + 2) * p->cellw[k] : p->L[k]);
+            cwmax = std::max(cwmax, p->cellw[k]);
+        }
+        const double e = 6.0 * u32 * qmax;
+        p->wslack = 2.0 * u32 * qmax + 1e-9 * cwmax;
+        for (auto& q : p->pp) {
+            if (q.r2_list < 0.0) {
+                q.l_lo32 = q.l_hi32 = -1.0f;
+                continue;
+            }
+            const double R = std::sqrt(q.r2_list);
+            const double beta = 4.0 * ((2.0 * std::sqrt(3.0) * e * R + 3.0 * e * e) / q.r2_list + 3.0 * u32);
+            const double lo = q.r2_list * (1.0 - beta), hi = q.r2_list * (1.0 + beta);
+            float flo = (float)lo, fhi = (float)hi;
+            if ((double)flo > lo) flo = std::nextafter(flo, -INFINITY);
+            if ((double)fhi < hi) fhi = std::nextafter(fhi, INFINITY);
+            q.l_lo32 = std::max(flo, 0.0f);
+            q.l_hi32 = fhi;
+        }
+    }
     p->npad = (int)((p->n + 63) / 64 * 64);
     int maxnb;
     if (d->neighbor_capacity)
@@ -315,9 +346,19 @@ struct pis_ctx {
     int n_atom_work = 0;
     WorkItem* d_tile_work = nullptr;
     int n_tile_work = 0;
-    int scap = 1024;
+    // tiles grouped by LDS staging capacity: one launch per class
+    struct TileClass {
+        int scap;
+        int first;
+        int count;
+    };
+    std::vector<TileClass> classes;
+    int scap = 1024;  // largest class (single-system helpers)
     bool mixed = false;
     bool multi = false;
+    // captured step sequences, keyed by whether thermo is recorded
+    bool use_graph = false;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
     void* d_xfer = nullptr;
     size_t xfer_bytes = 0;
     // statistics / timing
@@ -396,6 +437,7 @@ static pis_status alloc_system(SysHost& h)
         d.nt[k] = p.nt[k];
     }
     d.half_skin2 = 0.25 * p.skin * p.skin;
+    d.wslack = p.wslack;
     d.dt = 0.0;
     d.dt2 = 0.0;
     d.kb = p.kb;
@@ -455,8 +497,43 @@ static void free_ctx(pis_ctx* c)
     if (c->d_tile_work) (void)hipFree(c->d_tile_work);
     if (c->d_xfer) (void)hipFree(c->d_xfer);
     for (auto e : c->ev) (void)hipEventDestroy(e);
+    for (auto g : c->graph)
+        if (g) (void)hipGraphExecDestroy(g);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+// Groups every tile of every system by its LDS staging class (one launch per class) and
+// uploads the work table.  Captured step graphs refer to the old table and are dropped.
+static pis_status group_tiles(pis_ctx* c)
+{
+    std::map<int, std::vector<WorkItem>> by_class;
+    c->scap = 1024;
+    for (size_t k = 0; k < c->sys.size(); ++k) {
+        const Plan& p = c->sys[k].plan;
+        c->scap = std::max(c->scap, p.scap);
+        for (int t = 0; t < p.ntile; ++t) by_class[p.scap].push_back(WorkItem{(int32_t)k, t});
+    }
+    std::vector<WorkItem> tw;
+    c->classes.clear();
+    for (auto& kv : by_class) {
+        c->classes.push_back({kv.first, (int)tw.size(), (int)kv.second.size()});
+        tw.insert(tw.end(), kv.second.begin(), kv.second.end());
+    }
+    for (auto& g : c->graph) {
+        if (g) PIS_HIP(hipGraphExecDestroy(g));
+        g = nullptr;
+    }
+    if (c->d_tile_work) PIS_HIP(hipFree(c->d_tile_work));
+    c->d_tile_work = nullptr;
+    c->n_tile_work = (int)tw.size();
+    hipError_t e = hipMalloc(&c->d_tile_work, sizeof(WorkItem) * tw.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(c->d_tile_work, tw.data(), sizeof(WorkItem) * tw.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? PIS_ERR_OUT_OF_MEMORY : PIS_ERR_HIP,
+                    std::string("tile table: ") + hipGetErrorString(e));
+    return PIS_OK;
 }
 
 static pis_status create_ctx(const pis_system_desc* descs, uint32_t nsys, pis_ctx** out)
@@ -484,7 +561,7 @@ static pis_status create_ctx(const pis_system_desc* descs, uint32_t nsys, pis_ct
     }
     c->sys.resize(nsys);
     c->mixed = plans[0].mixed;
-    std::vector<WorkItem> aw, tw;
+    std::vector<WorkItem> aw;
     for (uint32_t k = 0; k < nsys; ++k) {
         c->sys[k].plan = std::move(plans[k]);
         pis_status s = alloc_system(c->sys[k]);
@@ -493,22 +570,22 @@ static pis_status create_ctx(const pis_system_desc* descs, uint32_t nsys, pis_ct
             return s;
         }
         const Plan& p = c->sys[k].plan;
-        c->scap = std::max(c->scap, p.scap);
         c->multi |= p.ntypes > 1;
         for (int64_t a = 0; a < p.n; a += kAtomThreads) aw.push_back(WorkItem{(int32_t)k, (int32_t)a});
-        for (int t = 0; t < p.ntile; ++t) tw.push_back(WorkItem{(int32_t)k, t});
     }
     c->n_atom_work = (int)aw.size();
-    c->n_tile_work = (int)tw.size();
     e = hipMalloc(&c->d_sys, sizeof(SysDev) * nsys);
     if (e == hipSuccess) e = hipMalloc(&c->d_atom_work, sizeof(WorkItem) * aw.size());
-    if (e == hipSuccess) e = hipMalloc(&c->d_tile_work, sizeof(WorkItem) * tw.size());
     if (e == hipSuccess) e = hipMemcpy(c->d_atom_work, aw.data(), sizeof(WorkItem) * aw.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(c->d_tile_work, tw.data(), sizeof(WorkItem) * tw.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         free_ctx(c);
         return fail(e == hipErrorOutOfMemory ? PIS_ERR_OUT_OF_MEMORY : PIS_ERR_HIP,
                     std::string("context allocation: ") + hipGetErrorString(e));
+    }
+    pis_status st = group_tiles(c);
+    if (st != PIS_OK) {
+        free_ctx(c);
+        return st;
     }
     pis_status s = upload_sys(c);
     if (s != PIS_OK) {
@@ -544,10 +621,22 @@ static pis_status write_ctl(pis_ctx* c, uint32_t k, const int32_t* ctl)
     return PIS_OK;
 }
 
+static void enqueue_rebuild(pis_ctx* c)
+{
+    launch_rebuild_sort(c->stream, c->d_sys, (int)c->sys.size(), c->d_atom_work, c->n_atom_work);
+    for (const auto& k : c->classes)
+        launch_build_list(c->stream, c->d_sys, c->d_tile_work + k.first, k.count, k.scap);
+}
+
+static void enqueue_force(pis_ctx* c, int integrate)
+{
+    for (const auto& k : c->classes)
+        launch_force(c->stream, c->d_sys, c->d_tile_work + k.first, k.count, k.scap, c->mixed, integrate);
+}
+
 static pis_status run_rebuild(pis_ctx* c)
 {
-    launch_rebuild(c->stream, c->d_sys, (int)c->sys.size(), c->d_atom_work, c->n_atom_work,
-                   c->d_tile_work, c->n_tile_work, c->scap, c->multi);
+    enqueue_rebuild(c);
     launch_flip(c->stream, c->d_sys, (int)c->sys.size());
     PIS_HIP(hipGetLastError());
     return PIS_OK;
@@ -596,6 +685,29 @@ static pis_status sys_set_state(pis_ctx* c, uint32_t k, const double* pos, const
     launch_load_aos(c->stream, c->d_sys, (int)k, (int)n, dp, vel ? dv : nullptr, force ? df : nullptr, dt);
     PIS_TRY(run_rebuild(c));
     PIS_HIP(hipStreamSynchronize(c->stream));
+    // The planned staging capacity assumes a uniform density.  A configuration with denser
+    // spots (a lattice whose planes sit on cell faces, a cluster) that fills a tile's halo
+    // to within 2 % + 16 atoms of the class gets a larger class, sized from the halo count
+    // just measured with the headroom the plan leaves; the rebuild is then repeated.  A
+    // state's order after sorting depends only on its positions, so sorting again is
+    // harmless.
+    PIS_TRY(read_ctl(c, k, ctl));
+    Plan& p = h.plan;
+    const int staged = ctl[CTL_MAXSTAGE];
+    if (p.mode != MODE_WHOLE && staged * 1.02 + 16 > p.scap) {
+        const int grown = scap_for(staged * 1.1 + 48);
+        if (grown < 0)
+            return fail(PIS_ERR_CAPACITY, "system " + std::to_string(k) + ": " + std::to_string(staged) +
+                                              " atoms in one tile's halo exceed the LDS staging limit");
+        p.scap = grown;
+        PIS_TRY(group_tiles(c));
+        int32_t again[CTL_COUNT] = {0};
+        again[CTL_CUR] = ctl[CTL_CUR];
+        again[CTL_REBUILD] = 1;
+        PIS_TRY(write_ctl(c, k, again));
+        PIS_TRY(run_rebuild(c));
+        PIS_HIP(hipStreamSynchronize(c->stream));
+    }
     return check_flags(c);
 }
 
@@ -646,7 +758,7 @@ static pis_status launch_force_timed(pis_ctx* c, int integrate)
         }
         PIS_HIP(hipEventRecord(c->ev[c->ev_used], c->stream));
     }
-    launch_force(c->stream, c->d_sys, c->d_tile_work, c->n_tile_work, c->scap, c->multi, c->mixed, integrate);
+    enqueue_force(c, integrate);
     if (c->timing) {
         PIS_HIP(hipEventRecord(c->ev[c->ev_used + 1], c->stream));
         c->ev_used += 2;
@@ -723,13 +835,32 @@ static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_
     PIS_TRY(upload_sys(c));
     const int ns = (int)c->sys.size();
     launch_set_ctl(c->stream, c->d_sys, ns, CTL_STEP, 0);
-    for (uint64_t t = 0; t < nsteps; ++t) {
-        launch_integrate_pos(c->stream, c->d_sys, c->d_atom_work, c->n_atom_work);
-        launch_rebuild(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work, c->d_tile_work,
-                       c->n_tile_work, c->scap, c->multi);
-        PIS_TRY(launch_force_timed(c, 1));
-        launch_reduce(c->stream, c->d_sys, ns, 1, out ? 1 : 0);
-        if ((t & 255) == 0) PIS_HIP(hipGetLastError());
+    const int rec = out ? 1 : 0;
+    if (c->use_graph && !c->timing) {
+        // one captured step (every kernel's arguments are step-invariant: the thermo
+        // record index and the live buffer set live on the device), replayed nsteps times
+        if (!c->graph[rec]) {
+            hipGraph_t g = nullptr;
+            PIS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            launch_integrate_pos(c->stream, c->d_sys, c->d_atom_work, c->n_atom_work);
+            enqueue_rebuild(c);
+            enqueue_force(c, 1);
+            launch_reduce(c->stream, c->d_sys, ns, 1, rec);
+            PIS_HIP(hipStreamEndCapture(c->stream, &g));
+            hipError_t e = hipGraphInstantiate(&c->graph[rec], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return fail(PIS_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+        }
+        for (uint64_t t = 0; t < nsteps; ++t) PIS_HIP(hipGraphLaunch(c->graph[rec], c->stream));
+        c->force_launches += nsteps * c->classes.size();
+    } else {
+        for (uint64_t t = 0; t < nsteps; ++t) {
+            launch_integrate_pos(c->stream, c->d_sys, c->d_atom_work, c->n_atom_work);
+            enqueue_rebuild(c);
+            PIS_TRY(launch_force_timed(c, 1));
+            launch_reduce(c->stream, c->d_sys, ns, 1, rec);
+            if ((t & 255) == 0) PIS_HIP(hipGetLastError());
+        }
     }
     PIS_HIP(hipGetLastError());
     PIS_HIP(hipStreamSynchronize(c->stream));
@@ -887,8 +1018,9 @@ pis_status pis_get_neighbor_list(pis_ctx* ctx, uint64_t* offsets, uint32_t* idx,
     const size_t n = (size_t)h.plan.n, mnb = (size_t)h.plan.maxnb;
     uint32_t* d_out = nullptr;
     PIS_HIP(hipMalloc(&d_out, (n * mnb + n) * 4));
-    launch_list_to_orig(ctx->stream, ctx->d_sys, ctx->d_tile_work, ctx->n_tile_work, ctx->scap,
-                        ctx->multi, d_out, d_out + n * mnb);
+    for (const auto& k : ctx->classes)
+        launch_list_to_orig(ctx->stream, ctx->d_sys, ctx->d_tile_work + k.first, k.count, k.scap, d_out,
+                            d_out + n * mnb);
     std::vector<uint32_t> buf(n * mnb + n);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
@@ -961,6 +1093,13 @@ pis_status pis_copy_calibration(uint64_t bytes, int32_t reps, double* ms_per_cop
     if (a) (void)hipFree(a);
     if (b) (void)hipFree(b);
     if (e != hipSuccess) return fail(PIS_ERR_HIP, std::string("copy calibration: ") + hipGetErrorString(e));
+    return PIS_OK;
+}
+
+pis_status pis_set_graph_mode(pis_ctx* ctx, int32_t enable)
+{
+    if (!ctx) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
+    ctx->use_graph = enable != 0;
     return PIS_OK;
 }
 

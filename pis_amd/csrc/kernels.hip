@@ -9,7 +9,7 @@
 //   k_bin             cell of every atom, integer histogram           (neighbour_list.rs:9-42)
 //   k_scan            exclusive scan of the histogram per system
 //   k_scatter         atoms into cell slots (atomic slot, then...)
-//   k_rank_permute    ...deterministic rank by original index inside the cell, and the
+//   k_rank_permute    ...deterministic rank by x (then original index) inside the cell, and the
 //                     permutation of all per-atom state into the other buffer set
 //   k_build_list      full neighbour list at rcut+skin, reference predicate bit-for-bit
 //                     (lennard_jones.rs:344-416)
@@ -32,6 +32,9 @@
 #endif
 #ifndef PIS_FORCE_MIN_WAVES_FP64
 #define PIS_FORCE_MIN_WAVES_FP64 1
+#endif
+#ifndef PIS_BUILD_UNROLL
+#define PIS_BUILD_UNROLL 4
 #endif
 #ifndef PIS_BUILD_GRID_CAP
 #define PIS_BUILD_GRID_CAP (1 << 30)
@@ -58,6 +61,12 @@ __device__ __forceinline__ double mi_fast(double d, double L, double Linv)
 {
     const double k = rint(d * Linv);
     return fma(-k, L, d);
+}
+
+__device__ __forceinline__ float mi_fast32(float d, float L, float Linv)
+{
+    const float k = rintf(d * Linv);
+    return fmaf(-k, L, d);
 }
 
 // apply_boundary_conditions_dis (simulation_box.rs:17-27) for an orthorhombic box:
@@ -144,10 +153,9 @@ enum { STAGE_RAW = 0, STAGE_REL = 1, STAGE_INDEX = 2 };
 // padding (list entries past an atom's count point there and are masked by count).
 // Returns the number of staged atoms (uniform over the block); > SCAP means overflow
 // and nothing was staged.
-template <int SCAP, int WHAT>
+template <int SCAP, int WHAT, typename T>
 __device__ int stage_tile(const SysDev& s, const State& st, const TileGeom& g, double cx, double cy,
-                          double cz, double* Px, double* Py, double* Pz, uint8_t* Pt, int* pref,
-                          int* ecell, int* stmp)
+                          double cz, T* Px, T* Py, T* Pz, uint8_t* Pt, int* pref, int* ecell, int* stmp)
 {
     const int tid = threadIdx.x;
     int cnt = 0;
@@ -193,16 +201,16 @@ __device__ int stage_tile(const SysDev& s, const State& st, const TileGeom& g, d
                 y = mi_fast(y - cy, s.L[1], s.Linv[1]);
                 z = mi_fast(z - cz, s.L[2], s.Linv[2]);
             }
-            Px[sl] = x;
-            Py[sl] = y;
-            Pz[sl] = z;
+            Px[sl] = (T)x;
+            Py[sl] = (T)y;
+            Pz[sl] = (T)z;
             Pt[sl] = st.type[a];
         }
     }
     if (tid == 0 && WHAT != STAGE_INDEX) {
-        Px[SCAP] = 0.0;
-        Py[SCAP] = 0.0;
-        Pz[SCAP] = 0.0;
+        Px[SCAP] = (T)0;
+        Py[SCAP] = (T)0;
+        Pz[SCAP] = (T)0;
         Pt[SCAP] = 0;
     }
     __syncthreads();
@@ -211,7 +219,7 @@ __device__ int stage_tile(const SysDev& s, const State& st, const TileGeom& g, d
 
 // Tile-local halo entry and LDS slot of own atom `a` (modes TILE_REL / RAW_MI).
 __device__ __forceinline__ void own_slot(const SysDev& s, const TileGeom& g, const int* pref, int a,
-                                         int* e0, int* slot)
+                                         int* e0, int* slot, int* ly_out = nullptr, int* lz_out = nullptr)
 {
     int lo = g.c0, hi = g.c1 - 1;
     while (lo < hi) {
@@ -225,6 +233,22 @@ __device__ __forceinline__ void own_slot(const SysDev& s, const TileGeom& g, con
     const int lx = k % g.sx, ly = (k / g.sx) % g.sy, lz = k / (g.sx * g.sy);
     *e0 = ((lz + 1) * (g.sy + 2) + (ly + 1)) * (g.sx + 2) + (lx + 1);
     *slot = pref[*e0] + (a - s.cell_start[lo]);
+    if (ly_out) *ly_out = ly;
+    if (lz_out) *lz_out = lz;
+}
+
+// First index in [lo, hi) of the ascending LDS run P with P[i] >= v (or > v for UPPER).
+template <bool UPPER>
+__device__ __forceinline__ int search_sorted(const float* P, int lo, int hi, double v)
+{
+    int n = hi - lo;
+    while (n > 0) {
+        const int h = n >> 1;
+        const bool right = UPPER ? P[lo + h] <= v : P[lo + h] < v;
+        lo = right ? lo + h + 1 : lo;
+        n = right ? n - h - 1 : h;
+    }
+    return lo;
 }
 
 // ---------------------------------------------------------------------------------
@@ -394,8 +418,9 @@ __global__ __launch_bounds__(kAtomThreads) void k_scatter(const SysDev* __restri
     }
 }
 
-// Inside a cell, atoms end up in ascending original index (the reference's rcut_cells
-// order), whatever order the atomics placed them in.
+// Inside a cell, atoms end up in ascending x (ties by original index), whatever order
+// the atomics placed them in: the order is a function of the positions alone, and a row
+// of cells along x is then sorted by x, which the list build's x windows rely on.
 __global__ __launch_bounds__(kAtomThreads) void k_rank_permute(const SysDev* __restrict__ sys,
                                                                 const WorkItem* __restrict__ work,
                                                                 int nwork)
@@ -412,10 +437,14 @@ __global__ __launch_bounds__(kAtomThreads) void k_rank_permute(const SysDev* __r
         const int sc = s.atom_cell[a];
         const int c0 = s.cell_start[sc], c1 = s.cell_start[sc + 1];
         const uint32_t key = s.scat_key[p];
-        int rank = 0;
-        for (int q = c0; q < c1; ++q) rank += s.scat_key[q] < key;
-        const int np = c0 + rank;
         const double x = src.x[a], y = src.y[a], z = src.z[a];
+        int rank = 0;
+        for (int q = c0; q < c1; ++q) {
+            const double xq = src.x[s.scat_src[q]];
+            const uint32_t kq = s.scat_key[q];
+            rank += (xq < x || (xq == x && kq < key)) ? 1 : 0;
+        }
+        const int np = c0 + rank;
         dst.x[np] = x;
         dst.y[np] = y;
         dst.z[np] = z;
@@ -487,14 +516,14 @@ __device__ __forceinline__ int wave_max(int v)
 // bit-exact reference predicate (exact_within, raw positions) decides.
 template <int SCAP, bool PAIRMI, bool MULTI>
 __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, const TileGeom& g,
-                                          const double* Px, const double* Py, const double* Pz,
+                                          const float* Px, const float* Py, const float* Pz,
                                           const uint8_t* Pt, const int* pref, const int* ecell,
                                           const PairParam* ppl, int total, bool active, int a,
                                           double cx, double cy, double cz)
 {
     const int nt = s.ntypes;
     double xi = 0.0, yi = 0.0, zi = 0.0;
-    int ti = 0, e0 = 0, self = -1;
+    int ti = 0, e0 = 0, self = -1, ly = 0, lz = 0;
     if (active) {
         xi = st.x[a];
         yi = st.y[a];
@@ -503,7 +532,7 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
         if (s.mode == MODE_WHOLE)
             self = a;
         else
-            own_slot(s, g, pref, a, &e0, &self);
+            own_slot(s, g, pref, a, &e0, &self, &ly, &lz);
     }
     double qx = xi, qy = yi, qz = zi;
     if (!PAIRMI) {
@@ -511,7 +540,28 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
         qy = mi_fast(yi - cy, s.L[1], s.Linv[1]);
         qz = mi_fast(zi - cz, s.L[2], s.Linv[2]);
     }
-    const double lo1 = ppl[0].l_lo, hi1 = ppl[0].l_hi, ex1 = ppl[0].r2_list;
+    const float qx32 = (float)qx, qy32 = (float)qy, qz32 = (float)qz;
+    const float L32[3] = {(float)s.L[0], (float)s.L[1], (float)s.L[2]};
+    const float Li32[3] = {(float)s.Linv[0], (float)s.Linv[1], (float)s.Linv[2]};
+    // FP32 list-band edges of this lane's type against each type, held in registers
+    float tlo[kMaxTypes], thi[kMaxTypes];
+    double reach2 = -1.0;  // no pair beyond this r^2 can enter the list
+#pragma unroll
+    for (int t = 0; t < kMaxTypes; ++t) {
+        const PairParam& p = ppl[MULTI ? ti * nt + min(t, nt - 1) : 0];
+        tlo[t] = p.l_lo32;
+        thi[t] = p.l_hi32;
+        reach2 = fmax(reach2, p.l_hi);
+    }
+    // Tile-relative staging keeps each 3-cell row sorted by x, so a row shrinks to the x
+    // window that can hold a partner: half-width sqrt(reach^2 - dy^2 - dz^2), with dy, dz
+    // the atom's distances to the row's cells (0 in its own y or z layer).  Slack of 1e-9
+    // cell widths absorbs binning and rounding at the cell faces; the window widens by
+    // twice the FP32 error of the staged x.
+    const double slack = 1e-9 * fmax(s.cellw[0], fmax(s.cellw[1], s.cellw[2]));
+    const double ylo = (ly - 0.5 * g.sy) * s.cellw[1], zlo = (lz - 0.5 * g.sz) * s.cellw[2];
+    const double gym = fmax(0.0, (qy - ylo) - slack), gyp = fmax(0.0, (ylo + s.cellw[1] - qy) - slack);
+    const double gzm = fmax(0.0, (qz - zlo) - slack), gzp = fmax(0.0, (zlo + s.cellw[2] - qz) - slack);
     uint64_t lo = 0, hi = 0;
     int cnt = 0;
     const int maxg = s.maxnb >> 3;
@@ -524,9 +574,23 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
             if (s.mode == MODE_WHOLE) {
                 re = total;
             } else {
-                const int e = e0 + ((r % 3) - 1) * hx + ((r / 3) - 1) * hxy;
+                const int oy = (r % 3) - 1, oz = (r / 3) - 1;
+                const int e = e0 + oy * hx + oz * hxy;
                 rb = pref[e - 1];
                 re = pref[e + 2];
+                if (!PAIRMI) {
+                    const double gy = oy < 0 ? gym : (oy > 0 ? gyp : 0.0);
+                    const double gz = oz < 0 ? gzm : (oz > 0 ? gzp : 0.0);
+                    const double w2 = reach2 - gy * gy - gz * gz;
+                    if (w2 < 0.0) {
+                        re = rb;
+                    } else {
+                        const double w = sqrt(w2) * (1.0 + 1e-9) + s.wslack;
+                        const int wb = search_sorted<false>(Px, rb, re, qx - w);
+                        re = search_sorted<true>(Px, wb, re, qx + w);
+                        rb = wb;
+                    }
+                }
             }
         }
         // uniform over the wave by construction; readfirstlane makes the loops scalar
@@ -535,45 +599,63 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
             const int kmax = min(64, rowmax - cb);
             uint32_t mlo = 0, mhi = 0;
             bool band = false;
-            for (int k = 0; k < kmax; ++k) {
-                const int sl = rb + cb + k;
-                const bool valid = sl < re && sl != self;
-                const int sli = valid ? sl : SCAP;
-                double dx = Px[sli] - qx, dy = Py[sli] - qy, dz = Pz[sli] - qz;
-                if (PAIRMI) {
-                    dx = mi_fast(dx, s.L[0], s.Linv[0]);
-                    dy = mi_fast(dy, s.L[1], s.Linv[1]);
-                    dz = mi_fast(dz, s.L[2], s.Linv[2]);
+            // PIS_BUILD_UNROLL candidates per trip: their LDS loads issue together and
+            // their distance chains interleave, which is what hides latency in a tile
+            // whose few waves cannot hide it for each other
+            for (int k = 0; k < kmax; k += PIS_BUILD_UNROLL) {
+                float dx[PIS_BUILD_UNROLL], dy[PIS_BUILD_UNROLL], dz[PIS_BUILD_UNROLL];
+                int tt[PIS_BUILD_UNROLL];
+                bool ok[PIS_BUILD_UNROLL];
+#pragma unroll
+                for (int j = 0; j < PIS_BUILD_UNROLL; ++j) {
+                    const int sl = rb + cb + k + j;
+                    ok[j] = k + j < kmax && sl < re && sl != self;
+                    const int sli = ok[j] ? sl : SCAP;
+                    dx[j] = Px[sli] - qx32;
+                    dy[j] = Py[sli] - qy32;
+                    dz[j] = Pz[sli] - qz32;
+                    tt[j] = MULTI ? (int)Pt[sli] : 0;
                 }
-                const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-                double blo = lo1, bhi = hi1;
-                if (MULTI) {
-                    const PairParam& p = ppl[ti * nt + Pt[sli]];
-                    blo = p.l_lo;
-                    bhi = p.l_hi;
+#pragma unroll
+                for (int j = 0; j < PIS_BUILD_UNROLL; ++j) {
+                    if (PAIRMI) {
+                        dx[j] = mi_fast32(dx[j], L32[0], Li32[0]);
+                        dy[j] = mi_fast32(dy[j], L32[1], Li32[1]);
+                        dz[j] = mi_fast32(dz[j], L32[2], Li32[2]);
+                    }
+                    const float r2 = fmaf(dx[j], dx[j], fmaf(dy[j], dy[j], dz[j] * dz[j]));
+                    float blo = tlo[0], bhi = thi[0];
+                    if (MULTI) {
+                        blo = tt[j] == 1 ? tlo[1] : blo;
+                        bhi = tt[j] == 1 ? thi[1] : bhi;
+                        blo = tt[j] == 2 ? tlo[2] : blo;
+                        bhi = tt[j] == 2 ? thi[2] : bhi;
+                        blo = tt[j] == 3 ? tlo[3] : blo;
+                        bhi = tt[j] == 3 ? thi[3] : bhi;
+                    }
+                    const bool in = ok[j] && r2 <= blo;
+                    band |= ok[j] && !in && r2 <= bhi;
+                    const uint32_t bit = in ? 1u : 0u;
+                    if (k + j < 32)
+                        mlo |= bit << (k + j);
+                    else
+                        mhi |= bit << (k + j - 32);
                 }
-                const bool in = valid && r2 <= blo;
-                band |= valid && !in && r2 <= bhi;
-                const uint32_t bit = in ? 1u : 0u;
-                if (k < 32)
-                    mlo |= bit << k;
-                else
-                    mhi |= bit << (k - 32);
             }
             if (__any(band)) {
                 // rare: settle the band candidates with the reference predicate
                 for (int k = 0; k < kmax && band; ++k) {
                     const int sl = rb + cb + k;
                     if (sl >= re || sl == self) continue;
-                    double dx = Px[sl] - qx, dy = Py[sl] - qy, dz = Pz[sl] - qz;
+                    float dx = Px[sl] - qx32, dy = Py[sl] - qy32, dz = Pz[sl] - qz32;
                     if (PAIRMI) {
-                        dx = mi_fast(dx, s.L[0], s.Linv[0]);
-                        dy = mi_fast(dy, s.L[1], s.Linv[1]);
-                        dz = mi_fast(dz, s.L[2], s.Linv[2]);
+                        dx = mi_fast32(dx, L32[0], Li32[0]);
+                        dy = mi_fast32(dy, L32[1], Li32[1]);
+                        dz = mi_fast32(dz, L32[2], Li32[2]);
                     }
-                    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                     const PairParam& p = ppl[MULTI ? ti * nt + Pt[sl] : 0];
-                    if (r2 > p.l_lo && r2 <= p.l_hi &&
+                    if (r2 > p.l_lo32 && r2 <= p.l_hi32 &&
                         exact_within(s, st, slot_atom(pref, ecell, g.ne, sl), xi, yi, zi, p.r2_list)) {
                         if (k < 32)
                             mlo |= 1u << k;
@@ -616,7 +698,8 @@ __global__ __launch_bounds__(kTileThreads) void k_build_list(const SysDev* __res
                                                               const WorkItem* __restrict__ tiles,
                                                               int ntw)
 {
-    __shared__ double Px[SCAP + 1], Py[SCAP + 1], Pz[SCAP + 1];
+    // FP32 staging: half the LDS of the force kernel's, so more tiles share a CU
+    __shared__ float Px[SCAP + 1], Py[SCAP + 1], Pz[SCAP + 1];
     __shared__ uint8_t Pt[SCAP + 1];
     __shared__ int pref[kMaxHalo + 1];
     __shared__ int ecell[kMaxHalo];
@@ -640,6 +723,7 @@ __global__ __launch_bounds__(kTileThreads) void k_build_list(const SysDev* __res
             total = stage_tile<SCAP, STAGE_REL>(s, st, g, cx, cy, cz, Px, Py, Pz, Pt, pref, ecell, stmp);
         else
             total = stage_tile<SCAP, STAGE_RAW>(s, st, g, cx, cy, cz, Px, Py, Pz, Pt, pref, ecell, stmp);
+        if (threadIdx.x == 0) atomicMax(&s.ctl[CTL_MAXSTAGE], total);
         if (total > SCAP) {
             if (threadIdx.x == 0) atomicOr(&s.ctl[CTL_OVF_STAGE], 1);
             __syncthreads();
@@ -959,7 +1043,7 @@ __global__ __launch_bounds__(kTileThreads) void k_list_to_orig(const SysDev* __r
     const SysDev& s = sys[w.sys];
     const State st = s.st[s.ctl[CTL_CUR]];
     const TileGeom g = tile_geom(s, w.idx);
-    const int total = stage_tile<SCAP, STAGE_INDEX>(s, st, g, 0.0, 0.0, 0.0, Px, nullptr, nullptr,
+    const int total = stage_tile<SCAP, STAGE_INDEX, double>(s, st, g, 0.0, 0.0, 0.0, Px, nullptr, nullptr,
                                                     nullptr, pref, ecell, stmp);
     if (total > SCAP) return;
     const int* Li = reinterpret_cast<const int*>(Px);
@@ -1043,11 +1127,9 @@ void launch_integrate_pos(hipStream_t st, const SysDev* d_sys, const WorkItem* a
     hipLaunchKernelGGL(k_integrate_pos, dim3(n_atom_work), dim3(kAtomThreads), 0, st, d_sys, atom_work);
 }
 
-void launch_rebuild(hipStream_t st, const SysDev* d_sys, int nsys, const WorkItem* atom_work,
-                    int n_atom_work, const WorkItem* tile_work, int n_tile_work, int scap,
-                    bool multi_type)
+void launch_rebuild_sort(hipStream_t st, const SysDev* d_sys, int nsys, const WorkItem* atom_work,
+                         int n_atom_work)
 {
-    (void)multi_type;
     // Rebuild kernels loop over their work items, so a modest grid covers any size and
     // costs little on the steps where they return at once.
     const int ga = grid_cap(n_atom_work, 512);
@@ -1056,6 +1138,12 @@ void launch_rebuild(hipStream_t st, const SysDev* d_sys, int nsys, const WorkIte
     hipLaunchKernelGGL(k_scatter, dim3(ga), dim3(kAtomThreads), 0, st, d_sys, atom_work, n_atom_work);
     hipLaunchKernelGGL(k_rank_permute, dim3(ga), dim3(kAtomThreads), 0, st, d_sys, atom_work,
                        n_atom_work);
+}
+
+void launch_build_list(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work, int n_tile_work,
+                       int scap)
+{
+    if (n_tile_work <= 0) return;
     const int gt = grid_cap(n_tile_work, PIS_BUILD_GRID_CAP);
 #define PIS_BUILD(S) \
     hipLaunchKernelGGL(k_build_list<S>, dim3(gt), dim3(kTileThreads), 0, st, d_sys, tile_work, n_tile_work)
@@ -1064,9 +1152,9 @@ void launch_rebuild(hipStream_t st, const SysDev* d_sys, int nsys, const WorkIte
 }
 
 void launch_force(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work, int n_tile_work,
-                  int scap, bool multi_type, bool mixed, int integrate)
+                  int scap, bool mixed, int integrate)
 {
-    (void)multi_type;
+    if (n_tile_work <= 0) return;
     const dim3 g(n_tile_work), b(kTileThreads);
 #define PIS_FORCE_M(S) \
     hipLaunchKernelGGL((k_force<S, true>), g, b, 0, st, d_sys, tile_work, n_tile_work, integrate)
@@ -1091,9 +1179,8 @@ void launch_flip(hipStream_t st, const SysDev* d_sys, int nsys)
 }
 
 void launch_list_to_orig(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work,
-                         int n_tile_work, int scap, bool multi_type, uint32_t* out, uint32_t* counts)
+                         int n_tile_work, int scap, uint32_t* out, uint32_t* counts)
 {
-    (void)multi_type;
     const dim3 g(n_tile_work), b(kTileThreads);
 #define PIS_L2O(S) \
     hipLaunchKernelGGL(k_list_to_orig<S>, g, b, 0, st, d_sys, tile_work, n_tile_work, out, counts)
@@ -1101,10 +1188,5 @@ void launch_list_to_orig(hipStream_t st, const SysDev* d_sys, const WorkItem* ti
 #undef PIS_L2O
 }
 
-size_t tile_lds_bytes(int scap, bool multi_type)
-{
-    (void)multi_type;
-    return (size_t)(scap + 1) * 25 + (kMaxHalo * 2 + 1) * 4;
-}
 
 }  // namespace pis

@@ -166,6 +166,9 @@ class Engine:
         check(lib().pis_get_stats(self._ptr, C.byref(s)))
         return {f[0]: getattr(s, f[0]) for f in _capi.Stats._fields_}
 
+    def set_graph_mode(self, enable: bool):
+        check(lib().pis_set_graph_mode(self._ptr, 1 if enable else 0))
+
     def set_kernel_timing(self, enable: bool):
         check(lib().pis_set_kernel_timing(self._ptr, 1 if enable else 0))
 

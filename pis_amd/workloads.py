@@ -51,6 +51,27 @@ def fcc_sites(nx: int, ny: int, nz: int, a: float) -> np.ndarray:
     return a * frac
 
 
+def box_lattice(n_atoms: int, L: float) -> np.ndarray:
+    """FCC sites (at least n_atoms of them) filling a cubic box of side L.  The cell counts
+    per axis may differ by a little: of the shapes holding enough sites, the one with the
+    largest nearest-site distance is taken, so that the surplus sites a cubic 4m^3 lattice
+    would need (up to ~2x the atoms just above 4m^3) do not squeeze atoms together."""
+    m0 = max(1, int(round((n_atoms / 4.0) ** (1.0 / 3.0))))
+    best = None
+    for mx in range(max(1, m0 - 2), m0 + 3):
+        for my in range(max(1, m0 - 2), mx + 1):
+            for mz in range(max(1, m0 - 2), my + 1):
+                if 4 * mx * my * mz < n_atoms:
+                    continue
+                a = (L / mx, L / my, L / mz)
+                d = min(np.hypot(a[0], a[1]), np.hypot(a[0], a[2]), np.hypot(a[1], a[2])) / 2.0
+                key = (d, -4 * mx * my * mz)
+                if best is None or key > best[0]:
+                    best = (key, (mx, my, mz))
+    mx, my, mz = best[1]
+    return fcc_sites(mx, my, mz, 1.0) * np.array([L / mx, L / my, L / mz])
+
+
 def wrap_positions(pos: np.ndarray, lengths) -> np.ndarray:
     L = np.asarray(lengths, dtype=np.float64)
     return pos - L * np.floor(pos / L)
@@ -147,12 +168,11 @@ KA_PAIRS = [  # Kob-Andersen 80:20 (eps, sigma); rcut = 2.5 sigma_ab
 
 def ka_mixture(n_atoms: int, density: float, temperature: float, seed: int, skin: float = 0.3,
                dt: float = 0.005, steps: int = 5000) -> SystemSpec:
-    """Kob-Andersen binary mixture: FCC sites of the smallest lattice holding n_atoms with
-    the surplus sites removed at random, 80 % type 1 and 20 % type 2 assigned at random."""
+    """Kob-Andersen binary mixture: FCC sites of box_lattice with the surplus sites removed
+    at random, 80 % type 1 and 20 % type 2 assigned at random."""
     rng = np.random.default_rng(seed)
-    m = int(np.ceil((n_atoms / 4.0) ** (1.0 / 3.0) - 1e-9))
     L = (n_atoms / density) ** (1.0 / 3.0)
-    sites = fcc_sites(m, m, m, L / m)
+    sites = box_lattice(n_atoms, L)
     keep = np.sort(rng.choice(sites.shape[0], size=n_atoms, replace=False))
     pos = wrap_positions(sites[keep], [L, L, L])
     nb = int(round(0.2 * n_atoms))
@@ -177,11 +197,10 @@ def ka_mixture(n_atoms: int, density: float, temperature: float, seed: int, skin
 
 def lj_box(n_atoms: int, density: float, temperature: float, seed: int, rcut: float = 2.5,
            skin: float = 0.3, dt: float = 0.005, steps: int = 1000) -> SystemSpec:
-    """Single-species LJ box of any atom count: FCC sites with surplus removed at random."""
+    """Single-species LJ box of any atom count: box_lattice sites, surplus removed at random."""
     rng = np.random.default_rng(seed)
-    m = int(np.ceil((n_atoms / 4.0) ** (1.0 / 3.0) - 1e-9))
     L = (n_atoms / density) ** (1.0 / 3.0)
-    sites = fcc_sites(m, m, m, L / m)
+    sites = box_lattice(n_atoms, L)
     keep = np.sort(rng.choice(sites.shape[0], size=n_atoms, replace=False))
     pos = wrap_positions(sites[keep], [L, L, L])
     vel = start_velocities(np.ones(n_atoms), temperature, 1.0, rng)

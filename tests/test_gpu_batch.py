@@ -1,0 +1,113 @@
+"""GPU tests of the batched path: many systems of uneven size in one context.
+
+Every kernel of a step covers all systems of a context at once (one launch per LDS
+staging class), and each system's arithmetic is the same whether it runs alone or in a
+batch, so a system's trajectory must be bit-identical in both; the same holds between
+the directly launched step and its replayed hipGraph.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from pis_amd import Engine, workloads as W
+from tests.conftest import force_err
+
+pytestmark = pytest.mark.gpu
+
+
+def small_ensemble():
+    """Eight KA systems spanning the thin-box (whole-box staging) and tiled modes, plus a
+    single-species box with another cutoff, so several staging classes share a batch."""
+    specs = W.ka_ensemble(8, seed=3, nmin=256, nmax=9000, steps=100)
+    specs.append(W.lj_box(3000, 0.9, 1.2, seed=21, rcut=3.0))
+    return specs
+
+
+def run(specs, precision, nsteps, graph=False):
+    with Engine(specs, precision) as eng:
+        eng.set_graph_mode(graph)
+        eng.load()
+        pe0 = eng.compute_potential()
+        th = eng.run_nve([s.dt for s in specs], nsteps)
+        states = [eng.get_state(k) for k in range(len(specs))]
+        st = eng.stats()
+    return pe0, th, states, st
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_equals_alone(precision):
+    specs = small_ensemble()
+    nsteps = 150
+    pe0, th, states, st = run(specs, precision, nsteps)
+    assert st["rebuilds"] > 0
+    for k, s in enumerate(specs):
+        print(f"system {k}: N={s.n}", flush=True)
+        a_pe0, a_th, a_states, _ = run([s], precision, nsteps)
+        assert a_pe0[0] == pe0[k]
+        assert np.array_equal(a_th[:, 0, :], th[:, k, :])
+        for a, b in zip(a_states[0], states[k]):
+            assert np.array_equal(a, b)
+
+
+def test_graph_replay_equals_direct_launches():
+    specs = small_ensemble()
+    nsteps = 200
+    _, th_d, st_d, _ = run(specs, "mixed", nsteps, graph=False)
+    _, th_g, st_g, stats = run(specs, "mixed", nsteps, graph=True)
+    assert stats["rebuilds"] > 0
+    assert np.array_equal(th_d, th_g)
+    for k in range(len(specs)):
+        for a, b in zip(st_d[k], st_g[k]):
+            assert np.array_equal(a, b)
+
+
+def test_graph_replays_across_calls_and_eager_work():
+    """Replays of one captured graph, interleaved with eager work (state reads, a force
+    evaluation) between calls, match the same sequence with direct launches."""
+    specs = small_ensemble()[:4]
+    out = []
+    for graph in (False, True):
+        with Engine(specs, "fp64") as eng:
+            eng.set_graph_mode(graph)
+            eng.load()
+            eng.compute_potential()
+            recs = []
+            for _ in range(4):
+                recs.append(eng.run_nve([s.dt for s in specs], 40))
+                recs.append(eng.compute_potential()[None, :])
+                recs.append(np.concatenate([np.ravel(a) for a in eng.get_state(1)]))
+            out.append(recs)
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
+def test_batched_fp64_forces_match_oracle():
+    specs = small_ensemble()
+    with Engine(specs, "fp64") as eng:
+        eng.load()
+        pe = eng.compute_potential()
+        forces = [eng.get_state(k)[2] for k in range(len(specs))]
+    for k, s in enumerate(specs):
+        o = po.OSystem.from_spec(s)
+        thin = any(int(L // (max(p[4] for p in s.pairs) + s.skin)) < 3 for L in s.box.lengths)
+        pe_ref = o.compute(po.MGR_BRUTE if thin else po.MGR_OFFSET)
+        print(f"system {k}: N={s.n} thin={thin}", flush=True)
+        assert force_err(forces[k], o.force) <= 1e-10
+        assert abs(pe[k] - pe_ref) <= 1e-10 * abs(pe_ref)
+
+
+def test_dense_halo_grows_staging_class():
+    """A KA box whose fcc planes sit on cell faces puts ~2,180 atoms into some tile halos,
+    past the 2,048-atom class its uniform-density plan picks: loading the state grows the
+    class from the measured halo counts, and forces still match the oracle."""
+    spec = W.ka_mixture(14365, 1.2, 1.0, seed=5)
+    with Engine(spec, "fp64") as eng:
+        eng.load()
+        pe = float(eng.compute_potential()[0])
+        f = eng.get_state(0)[2]
+        th = eng.run_nve(spec.dt, 50)
+    o = po.OSystem.from_spec(spec)
+    pe_ref = o.compute(po.MGR_OFFSET)
+    assert force_err(f, o.force) <= 1e-10
+    assert abs(pe - pe_ref) <= 1e-10 * abs(pe_ref)
+    assert np.all(np.isfinite(th))
