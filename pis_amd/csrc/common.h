@@ -21,6 +21,7 @@ constexpr int kAtomThreads = 256;  // workgroup of the per-atom kernels
 constexpr int kMaxHalo = 256;      // halo-box cells a tile may stage
 constexpr int kMaxTypes = 4;
 constexpr int kMaxList = 512;      // neighbour-list capacity limit per atom
+constexpr int kMaxLanes = 4;       // streams a context spreads its staging classes over
 
 // ctl[] slots (device ints per system)
 enum {

@@ -351,6 +351,7 @@ struct pis_ctx {
         int scap;
         int first;
         int count;
+        int lane;  // 0: `stream`, k > 0: side[k-1]
     };
     std::vector<TileClass> classes;
     int scap = 1024;  // largest class (single-system helpers)
@@ -359,6 +360,12 @@ struct pis_ctx {
     // captured step sequences, keyed by whether thermo is recorded
     bool use_graph = false;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // staging classes after the first run their list build and force on streams of their
+    // own (default priority), forked from and joined back into `stream` by events
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> join;
+    hipEvent_t fork = nullptr;
+    int nlanes = 1;
     void* d_xfer = nullptr;
     size_t xfer_bytes = 0;
     // statistics / timing
@@ -499,6 +506,9 @@ static void free_ctx(pis_ctx* c)
     for (auto e : c->ev) (void)hipEventDestroy(e);
     for (auto g : c->graph)
         if (g) (void)hipGraphExecDestroy(g);
+    for (auto x : c->side) (void)hipStreamDestroy(x);
+    for (auto x : c->join) (void)hipEventDestroy(x);
+    if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -517,12 +527,39 @@ static pis_status group_tiles(pis_ctx* c)
     std::vector<WorkItem> tw;
     c->classes.clear();
     for (auto& kv : by_class) {
-        c->classes.push_back({kv.first, (int)tw.size(), (int)kv.second.size()});
+        c->classes.push_back({kv.first, (int)tw.size(), (int)kv.second.size(), 0});
         tw.insert(tw.end(), kv.second.begin(), kv.second.end());
+    }
+    // Classes share kMaxLanes streams (the device serves a few hardware queues per
+    // process; more streams would only time-slice): largest estimated work first, each
+    // onto the least loaded stream.
+    {
+        std::vector<int> order(c->classes.size());
+        for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
+        auto work = [&](int k) { return (double)c->classes[k].count * c->classes[k].scap; };
+        std::sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
+        const int nl = (int)std::min<size_t>(kMaxLanes, c->classes.size());
+        std::vector<double> load(nl, 0.0);
+        for (int k : order) {
+            const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            c->classes[k].lane = l;
+            load[l] += work(k);
+        }
+        c->nlanes = nl;
     }
     for (auto& g : c->graph) {
         if (g) PIS_HIP(hipGraphExecDestroy(g));
         g = nullptr;
+    }
+    if (!c->fork && c->nlanes > 1)
+        PIS_HIP(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
+    while ((int)c->side.size() + 1 < c->nlanes) {
+        hipStream_t x = nullptr;
+        hipEvent_t j = nullptr;
+        PIS_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+        c->side.push_back(x);
+        PIS_HIP(hipEventCreateWithFlags(&j, hipEventDisableTiming));
+        c->join.push_back(j);
     }
     if (c->d_tile_work) PIS_HIP(hipFree(c->d_tile_work));
     c->d_tile_work = nullptr;
@@ -621,22 +658,39 @@ static pis_status write_ctl(pis_ctx* c, uint32_t k, const int32_t* ctl)
     return PIS_OK;
 }
 
-static void enqueue_rebuild(pis_ctx* c)
+// The per-class part of a step: list build (if `build`) then forces (if `force >= 0`,
+// with that integrate flag).  Classes touch disjoint tiles and atoms, so the classes of
+// lane l > 0 run on side[l-1], forked after everything already on `stream` and joined
+// before anything enqueued later; captured into a graph, the fork becomes parallel
+// branches.
+static pis_status enqueue_classes(pis_ctx* c, bool build, int force)
 {
-    launch_rebuild_sort(c->stream, c->d_sys, (int)c->sys.size(), c->d_atom_work, c->n_atom_work);
-    for (const auto& k : c->classes)
-        launch_build_list(c->stream, c->d_sys, c->d_tile_work + k.first, k.count, k.scap);
+    const int nl = c->nlanes;
+    if (nl > 1) {
+        PIS_HIP(hipEventRecord(c->fork, c->stream));
+        for (int l = 1; l < nl; ++l) PIS_HIP(hipStreamWaitEvent(c->side[l - 1], c->fork, 0));
+    }
+    for (const auto& q : c->classes) {
+        hipStream_t st = q.lane == 0 ? c->stream : c->side[q.lane - 1];
+        if (build) launch_build_list(st, c->d_sys, c->d_tile_work + q.first, q.count, q.scap);
+        if (force >= 0) launch_force(st, c->d_sys, c->d_tile_work + q.first, q.count, q.scap, c->mixed, force);
+    }
+    for (int l = 1; l < nl; ++l) {
+        PIS_HIP(hipEventRecord(c->join[l - 1], c->side[l - 1]));
+        PIS_HIP(hipStreamWaitEvent(c->stream, c->join[l - 1], 0));
+    }
+    return PIS_OK;
 }
 
-static void enqueue_force(pis_ctx* c, int integrate)
+static pis_status enqueue_rebuild(pis_ctx* c)
 {
-    for (const auto& k : c->classes)
-        launch_force(c->stream, c->d_sys, c->d_tile_work + k.first, k.count, k.scap, c->mixed, integrate);
+    launch_rebuild_sort(c->stream, c->d_sys, (int)c->sys.size(), c->d_atom_work, c->n_atom_work);
+    return enqueue_classes(c, true, -1);
 }
 
 static pis_status run_rebuild(pis_ctx* c)
 {
-    enqueue_rebuild(c);
+    PIS_TRY(enqueue_rebuild(c));
     launch_flip(c->stream, c->d_sys, (int)c->sys.size());
     PIS_HIP(hipGetLastError());
     return PIS_OK;
@@ -758,7 +812,7 @@ static pis_status launch_force_timed(pis_ctx* c, int integrate)
         }
         PIS_HIP(hipEventRecord(c->ev[c->ev_used], c->stream));
     }
-    enqueue_force(c, integrate);
+    PIS_TRY(enqueue_classes(c, false, integrate));
     if (c->timing) {
         PIS_HIP(hipEventRecord(c->ev[c->ev_used + 1], c->stream));
         c->ev_used += 2;
@@ -843,21 +897,33 @@ static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_
             hipGraph_t g = nullptr;
             PIS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             launch_integrate_pos(c->stream, c->d_sys, c->d_atom_work, c->n_atom_work);
-            enqueue_rebuild(c);
-            enqueue_force(c, 1);
+            launch_rebuild_sort(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work);
+            pis_status cs = enqueue_classes(c, true, 1);
             launch_reduce(c->stream, c->d_sys, ns, 1, rec);
-            PIS_HIP(hipStreamEndCapture(c->stream, &g));
+            hipError_t ce = hipStreamEndCapture(c->stream, &g);
+            if (cs != PIS_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return cs;
+            }
+            if (ce != hipSuccess) return fail(PIS_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
             hipError_t e = hipGraphInstantiate(&c->graph[rec], g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             if (e != hipSuccess) return fail(PIS_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
         }
         for (uint64_t t = 0; t < nsteps; ++t) PIS_HIP(hipGraphLaunch(c->graph[rec], c->stream));
-        c->force_launches += nsteps * c->classes.size();
+        c->force_launches += nsteps;
     } else {
         for (uint64_t t = 0; t < nsteps; ++t) {
             launch_integrate_pos(c->stream, c->d_sys, c->d_atom_work, c->n_atom_work);
-            enqueue_rebuild(c);
-            PIS_TRY(launch_force_timed(c, 1));
+            launch_rebuild_sort(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work);
+            if (c->timing) {
+                // timed: every list build first, then the force launches between events
+                PIS_TRY(enqueue_classes(c, true, -1));
+                PIS_TRY(launch_force_timed(c, 1));
+            } else {
+                PIS_TRY(enqueue_classes(c, true, 1));
+                c->force_launches++;
+            }
             launch_reduce(c->stream, c->d_sys, ns, 1, rec);
             if ((t & 255) == 0) PIS_HIP(hipGetLastError());
         }
