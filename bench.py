@@ -159,6 +159,78 @@ def run_ensemble(steps, warmup, direct_steps):
     return out
 
 
+def serving_jobs(n_jobs, seed=2028):
+    """Serving stream: n_jobs seeded single-species LJ boxes, atoms log-uniform on
+    [500, 200000], density U(0.7, 1.1), T* U(0.7, 1.5), steps log-uniform on [100, 5000],
+    Poisson arrivals at 50 jobs/s (exponential gaps)."""
+    from pis_amd import workloads as W
+
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / 50.0, size=n_jobs)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    specs = []
+    for k in range(n_jobs):
+        n = int(np.exp(rng.uniform(np.log(500), np.log(200000))))
+        rho = float(rng.uniform(0.7, 1.1))
+        temp = float(rng.uniform(0.7, 1.5))
+        steps = int(np.exp(rng.uniform(np.log(100), np.log(5000))))
+        spec = W.lj_box(n, rho, temp, seed=seed * 1000 + k)
+        spec.steps = steps
+        specs.append(spec)
+    return specs, arrivals
+
+
+def run_serving(n_jobs, budget_gib=64, round_steps=50):
+    """Submits the stream at its arrival times (a feeder thread) to one mixed-precision
+    server and waits for every job; latency is completion minus submission."""
+    import threading
+
+    from pis_amd import Server
+
+    specs, arrivals = serving_jobs(n_jobs)
+    ids = [None] * n_jobs
+    with Server(hbm_budget_bytes=budget_gib << 30, max_systems=256, round_steps=round_steps,
+                precision="mixed") as srv:
+        t0 = time.perf_counter()
+
+        def feeder():
+            for k, s in enumerate(specs):
+                delay = arrivals[k] - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                ids[k] = srv.submit(s)
+
+        th = threading.Thread(target=feeder)
+        th.start()
+        th.join()
+        res = [srv.wait(j, state=False) for j in ids]
+        st = srv.stats()
+    bad = [r for r in res if r.status != 0]
+    if bad:
+        raise RuntimeError(f"{len(bad)} serving jobs failed: {bad[0].error}")
+    lat = np.array([r.latency_s for r in res])
+    first = min(r.submit_s for r in res)
+    last = max(r.done_s for r in res)
+    work = sum(r.n_atoms * r.steps for r in res)
+    return {
+        "jobs": n_jobs,
+        "arrival_rate_jobs_per_s": 50,
+        "p50_s": round(float(np.percentile(lat, 50)), 4),
+        "p99_s": round(float(np.percentile(lat, 99)), 4),
+        "mean_s": round(float(lat.mean()), 4),
+        "max_s": round(float(lat.max()), 4),
+        "throughput": round(work / (last - first), 1),
+        "unit": "atom-steps/s",
+        "makespan_s": round(last - first, 3),
+        "atom_steps": int(work),
+        "peak_resident_jobs": st["peak_resident"],
+        "peak_bytes": st["peak_bytes"],
+        "budget_bytes": budget_gib << 30,
+        "round_steps": round_steps,
+        "precision": "mixed",
+    }
+
+
 def cpu_baseline(spec, steps):
     """The oracle (faithful LJVOffsetManager + verlet_step_nve, 1 core) on `steps` steps
     of the same system; also returns its initial forces for the force-error figures."""
@@ -207,6 +279,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--ensemble-steps", type=int, default=5000,
                     help="steps of the 512-system batched ensemble (0: skip)")
+    ap.add_argument("--serve-jobs", type=int, default=100,
+                    help="jobs of the Poisson serving stream (0: skip)")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30,
                     help="bytes of the streaming-copy calibration run after the timed runs (0: skip)")
     args = ap.parse_args()
@@ -237,6 +311,13 @@ def main():
         ensemble = run_ensemble(args.ensemble_steps, 100, min(args.ensemble_steps, 1000))
         log(f"ensemble: {ensemble['total_atoms']} atoms, {ensemble['value']:.4e} atom-steps/s "
             f"(direct launches {ensemble.get('direct_launch_value', 0):.4e})")
+
+    serving = None
+    if args.serve_jobs:
+        log(f"serving: {args.serve_jobs} Poisson jobs at 50/s, continuous batching")
+        serving = run_serving(args.serve_jobs)
+        log(f"serving: p50 {serving['p50_s']} s, p99 {serving['p99_s']} s, "
+            f"{serving['throughput']:.4e} atom-steps/s over {serving['makespan_s']} s")
 
     copy_gbs = None
     if args.copy_bytes:
@@ -329,7 +410,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched_ensemble": ensemble,
-        "job_latency": None,
+        "job_latency": serving,
     }
     res.update(out_extra)
     print(json.dumps(res), flush=True)

@@ -50,7 +50,8 @@ typedef enum pis_status {
     PIS_ERR_OUT_OF_MEMORY = 8,
     PIS_ERR_NO_DEVICE = 9,
     PIS_ERR_CAPACITY = 10, /* neighbour/staging capacity still exceeded after regrowth */
-    PIS_ERR_NOT_FOUND = 11
+    PIS_ERR_NOT_FOUND = 11,
+    PIS_ERR_TIMEOUT = 12
 } pis_status;
 
 typedef enum pis_precision {
@@ -190,6 +191,44 @@ pis_status pis_batch_compute_potential(pis_ctx* ctx, double* pe_out);
  * [nsteps][n_systems] records or NULL. */
 pis_status pis_batch_verlet_step_nve(pis_ctx* ctx, const double* dts, uint64_t nsteps,
                                      pis_thermo* thermo_out);
+
+/* ---- Serving: continuous batching of NVE jobs on one GPU (no reference counterpart; the
+ * reference runs one Simulation::run per process, simulation.rs:33-105).  A worker thread
+ * keeps the admitted jobs' state resident and advances all of them together in rounds
+ * of `round_steps` batched steps; between rounds it retires finished jobs and admits
+ * queued ones in arrival order while `hbm_budget_bytes` and `max_systems` allow.  A job's
+ * result is bit-identical to the same job run alone (pis_ctx_create + compute_potential +
+ * verlet_step_nve with the same precision). */
+typedef struct pis_server pis_server;
+typedef struct {
+    uint64_t job_id;
+    int32_t status; /* PIS_OK, or the error that ended the job (pis_last_error has text) */
+    int32_t reserved;
+    uint64_t n_atoms;
+    uint64_t steps;
+    double submit_s, admit_s, done_s; /* seconds since pis_server_create */
+    pis_thermo last;                  /* thermo of the job's final step */
+} pis_job_result;
+typedef struct {
+    uint64_t submitted, completed, failed, rounds, atom_steps;
+    uint64_t resident, pending, bytes_in_use, peak_bytes;
+    uint32_t peak_resident, reserved;
+} pis_server_stats;
+
+pis_status pis_server_create(uint64_t hbm_budget_bytes, uint32_t max_systems, uint32_t round_steps,
+                             int32_t precision, pis_server** out);
+/* Queues a job (arrays are copied; vel may be NULL for zero velocities).  The job's
+ * desc->precision must equal the server's.  PIS_ERR_OUT_OF_MEMORY if the job alone exceeds
+ * the budget. */
+pis_status pis_server_submit(pis_server* srv, const pis_system_desc* desc, const double* pos,
+                             const double* vel, double dt, uint64_t nsteps, uint64_t* job_id);
+/* Blocks until the job ends (timeout_s < 0: no limit; PIS_ERR_TIMEOUT otherwise), then
+ * collects it: result (may be NULL) and final pos/vel/force [3n] (each may be NULL). */
+pis_status pis_server_wait(pis_server* srv, uint64_t job_id, double timeout_s, pis_job_result* result,
+                           double* pos, double* vel, double* force);
+pis_status pis_server_get_stats(pis_server* srv, pis_server_stats* out);
+/* Stops the worker; queued and running jobs end with an error. */
+pis_status pis_server_destroy(pis_server* srv);
 
 #ifdef __cplusplus
 }

@@ -7,9 +7,12 @@ that needs it does, and fails loudly if it has not been built.
 from ._capi import PisError, lib  # noqa: F401
 from .box import SimulationBox, box_inverse  # noqa: F401
 from .engine import Atoms, Engine, LennardJones, LJGpuManager, plan, thermo_to_array  # noqa: F401
+from .serve import JobResult, Server  # noqa: F401
 from .workloads import SystemSpec  # noqa: F401
 
 __all__ = [
+    "Server",
+    "JobResult",
     "Atoms",
     "Engine",
     "LennardJones",

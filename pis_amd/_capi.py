@@ -25,6 +25,7 @@ STATUS_NAMES = {
     9: "PIS_ERR_NO_DEVICE",
     10: "PIS_ERR_CAPACITY",
     11: "PIS_ERR_NOT_FOUND",
+    12: "PIS_ERR_TIMEOUT",
 }
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
@@ -101,6 +102,36 @@ class Stats(C.Structure):
     ]
 
 
+class JobResult(C.Structure):
+    _fields_ = [
+        ("job_id", C.c_uint64),
+        ("status", C.c_int32),
+        ("reserved", C.c_int32),
+        ("n_atoms", C.c_uint64),
+        ("steps", C.c_uint64),
+        ("submit_s", C.c_double),
+        ("admit_s", C.c_double),
+        ("done_s", C.c_double),
+        ("last", Thermo),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64),
+        ("completed", C.c_uint64),
+        ("failed", C.c_uint64),
+        ("rounds", C.c_uint64),
+        ("atom_steps", C.c_uint64),
+        ("resident", C.c_uint64),
+        ("pending", C.c_uint64),
+        ("bytes_in_use", C.c_uint64),
+        ("peak_bytes", C.c_uint64),
+        ("peak_resident", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
 # name -> (restype, argtypes); every function declared in include/pis_hip.h
 _P = C.c_void_p
 _DP = C.POINTER(C.c_double)
@@ -131,6 +162,14 @@ SIGNATURES = {
     "pis_sys_get_state": (C.c_int, [_P, C.c_uint32, _DP, _DP, _DP]),
     "pis_batch_compute_potential": (C.c_int, [_P, _DP]),
     "pis_batch_verlet_step_nve": (C.c_int, [_P, _DP, C.c_uint64, C.POINTER(Thermo)]),
+    "pis_server_create": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(_P)]),
+    "pis_server_submit": (
+        C.c_int,
+        [_P, C.POINTER(SystemDesc), _DP, _DP, C.c_double, C.c_uint64, C.POINTER(C.c_uint64)],
+    ),
+    "pis_server_wait": (C.c_int, [_P, C.c_uint64, C.c_double, C.POINTER(JobResult), _DP, _DP, _DP]),
+    "pis_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
+    "pis_server_destroy": (C.c_int, [_P]),
 }
 
 _lib = None

@@ -34,7 +34,8 @@ enum {
     CTL_NREBUILD = 6,  // builds performed
     CTL_STEP = 7,      // thermo record index
     CTL_MAXSTAGE = 8,  // most atoms staged by one tile since the state was set
-    CTL_COUNT = 9
+    CTL_LEFT = 9,      // NVE steps still to run: < 0 unlimited, 0 frozen (steps skip it)
+    CTL_COUNT = 10
 };
 
 enum { MODE_TILE_REL = 0, MODE_RAW_MI = 1, MODE_WHOLE = 2 };

@@ -1,0 +1,134 @@
+// engine.h -- internals of the engine shared by its translation units (engine.hip: plans,
+// contexts, the step loop and the C ABI; server.hip: continuous batching of jobs).
+#pragma once
+#include <string>
+#include <vector>
+
+#include "pis_hip.h"
+#include "common.h"
+#include "kernels.h"
+
+namespace pis {
+
+// Records `msg` as the calling thread's last error and returns `s`.
+pis_status fail(pis_status s, const std::string& msg);
+const std::string& last_error();
+
+#define PIS_HIP(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(e_ == hipErrorOutOfMemory ? PIS_ERR_OUT_OF_MEMORY : PIS_ERR_HIP,     \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+
+#define PIS_TRY(expr)                    \
+    do {                                 \
+        pis_status s_ = (expr);          \
+        if (s_ != PIS_OK) return s_;     \
+    } while (0)
+
+// Everything the host decides about one system before touching the device.
+struct Plan {
+    int64_t n = 0;
+    int ntypes = 0;
+    int nc[3] = {1, 1, 1}, tile[3] = {1, 1, 1}, nt[3] = {1, 1, 1};
+    int ncell = 1, ntile = 1, mode = MODE_WHOLE, maxnb = 8, scap = 1024, npad = 64;
+    double L[3] = {0, 0, 0}, Linv[3] = {0, 0, 0}, cellw[3] = {0, 0, 0};
+    double rcmax = 0, rcl = 0, skin = 0, density = 0, volume = 0, kb = 1, wslack = 0;
+    bool mixed = false;
+    std::vector<PairParam> pp;
+    std::vector<double> mass;
+    std::vector<uint8_t> type0;
+};
+
+
+pis_status make_plan(const pis_system_desc* d, Plan* p);
+// Smallest LDS staging class holding `need` atoms (-1 if none does).
+int stage_class_for(double need);
+
+struct SysHost {
+    Plan plan;
+    void* mem = nullptr;
+    size_t mem_bytes = 0;
+    ThermoDev* thermo = nullptr;
+    int thermo_cap = 0;
+    SysDev dev{};
+};
+
+}  // namespace pis
+
+struct pis_ctx {
+    using SysHost = pis::SysHost;
+    using WorkItem = pis::WorkItem;
+    using SysDev = pis::SysDev;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<SysHost> sys;
+    SysDev* d_sys = nullptr;
+    size_t sys_cap = 0;  // SysDev slots allocated at d_sys
+    WorkItem* d_atom_work = nullptr;
+    int n_atom_work = 0;
+    WorkItem* d_tile_work = nullptr;
+    int n_tile_work = 0;
+    // tiles grouped by LDS staging capacity: one launch per class
+    struct TileClass {
+        int scap;
+        int first;
+        int count;
+        int lane;  // 0: `stream`, k > 0: side[k-1]
+    };
+    std::vector<TileClass> classes;
+    int scap = 1024;  // largest class (single-system helpers)
+    bool mixed = false;
+    bool multi = false;
+    // captured step sequences, keyed by whether thermo is recorded
+    bool use_graph = false;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // staging classes after the first run their list build and force on streams of their
+    // own (default priority), forked from and joined back into `stream` by events
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> join;
+    hipEvent_t fork = nullptr;
+    int nlanes = 1;
+    void* d_xfer = nullptr;
+    size_t xfer_bytes = 0;
+    // statistics / timing
+    uint64_t steps = 0, force_launches = 0;
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    size_t ev_used = 0;
+    double force_ms = 0.0;
+    uint64_t timed_launches = 0;
+};
+
+namespace pis {
+
+// Device memory of one system (one allocation, carved) and its tables.
+pis_status alloc_system(SysHost& h);
+// Every system's descriptor to the device copy the kernels read.
+pis_status upload_sys(pis_ctx* c);
+void free_ctx(pis_ctx* c);
+// Tiles of all systems grouped by staging class, one table upload; drops captured graphs.
+pis_status group_tiles(pis_ctx* c);
+pis_status ensure_xfer(pis_ctx* c, size_t bytes);
+pis_status read_ctl(pis_ctx* c, uint32_t k, int32_t* ctl);
+pis_status write_ctl(pis_ctx* c, uint32_t k, const int32_t* ctl);
+pis_status enqueue_classes(pis_ctx* c, bool build, int force);
+pis_status enqueue_rebuild(pis_ctx* c);
+pis_status run_rebuild(pis_ctx* c);
+// Status of every system (or only system `only`) from its device flags.
+pis_status check_flags(pis_ctx* c, int only = -1);
+pis_status ensure_thermo(pis_ctx* c, int cap);
+pis_status sys_set_state(pis_ctx* c, uint32_t k, const double* pos, const double* vel, const double* force);
+pis_status sys_get_state(pis_ctx* c, uint32_t k, double* pos, double* vel, double* force);
+// An empty context (systems are added later) with the given precision.
+pis_status ctx_create_empty(bool mixed, pis_ctx** out);
+// Appends one planned system (device memory allocated, tables rebuilt) as index size()-1.
+pis_status ctx_add(pis_ctx* c, Plan&& plan);
+// Removes the systems at the given indices (any order); later indices shift down.
+pis_status ctx_remove(pis_ctx* c, std::vector<size_t> idx);
+// Device bytes a system with this plan occupies (its allocation plus `thermo_records`).
+size_t system_bytes(const Plan& p, int thermo_records);
+
+}  // namespace pis

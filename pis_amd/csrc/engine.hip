@@ -8,9 +8,7 @@
 #include <string>
 #include <vector>
 
-#include "pis_hip.h"
-#include "common.h"
-#include "kernels.h"
+#include "engine.h"
 
 using namespace pis;
 
@@ -19,25 +17,13 @@ using namespace pis;
 
 static thread_local std::string g_last_error;
 
-static pis_status fail(pis_status s, const std::string& msg)
+pis_status pis::fail(pis_status s, const std::string& msg)
 {
     g_last_error = msg;
     return s;
 }
 
-#define PIS_HIP(expr)                                                                        \
-    do {                                                                                     \
-        hipError_t e_ = (expr);                                                              \
-        if (e_ != hipSuccess)                                                                \
-            return fail(e_ == hipErrorOutOfMemory ? PIS_ERR_OUT_OF_MEMORY : PIS_ERR_HIP,     \
-                        std::string(#expr) + ": " + hipGetErrorString(e_));                  \
-    } while (0)
-
-#define PIS_TRY(expr)                    \
-    do {                                 \
-        pis_status s_ = (expr);          \
-        if (s_ != PIS_OK) return s_;     \
-    } while (0)
+const std::string& pis::last_error() { return g_last_error; }
 
 // ---------------------------------------------------------------------------------
 // planning (host only)
@@ -60,19 +46,6 @@ double r2max_for(double rc)
     }
     return r2;
 }
-
-struct Plan {
-    int64_t n = 0;
-    int ntypes = 0;
-    int nc[3] = {1, 1, 1}, tile[3] = {1, 1, 1}, nt[3] = {1, 1, 1};
-    int ncell = 1, ntile = 1, mode = MODE_WHOLE, maxnb = 8, scap = 1024, npad = 64;
-    double L[3] = {0, 0, 0}, Linv[3] = {0, 0, 0}, cellw[3] = {0, 0, 0};
-    double rcmax = 0, rcl = 0, skin = 0, density = 0, volume = 0, kb = 1, wslack = 0;
-    bool mixed = false;
-    std::vector<PairParam> pp;
-    std::vector<double> mass;
-    std::vector<uint8_t> type0;
-};
 
 int scap_for(double need)
 {
@@ -127,7 +100,7 @@ void choose_tiles(Plan& p)
     for (int d = 0; d < 3; ++d) p.tile[d] = bt[d];
 }
 
-pis_status make_plan(const pis_system_desc* d, Plan* p)
+pis_status make_plan_impl(const pis_system_desc* d, Plan* p)
 {
     if (!d) return fail(PIS_ERR_INVALID_ARGUMENT, "null system descriptor");
     if (d->n_atoms == 0) return fail(PIS_ERR_NO_ATOMS, "No atoms defined in input file");
@@ -325,84 +298,72 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 }  // namespace
 
+pis_status pis::make_plan(const pis_system_desc* d, Plan* p) { return make_plan_impl(d, p); }
+int pis::stage_class_for(double need) { return scap_for(need); }
+
 // ---------------------------------------------------------------------------------
 // context
 
-struct SysHost {
-    Plan plan;
-    void* mem = nullptr;
-    size_t mem_bytes = 0;
-    ThermoDev* thermo = nullptr;
-    int thermo_cap = 0;
-    SysDev dev{};
+namespace {
+
+// Carving of one system's allocation (offsets of every array, 256-byte aligned).
+struct Layout {
+    size_t st[2][11];
+    size_t r, ctl, cc, cf, cs, n2s, tc, ac, ss, sk, nbr, nnb, part, pp, m;
+    size_t total;
 };
 
-struct pis_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::vector<SysHost> sys;
-    SysDev* d_sys = nullptr;
-    WorkItem* d_atom_work = nullptr;
-    int n_atom_work = 0;
-    WorkItem* d_tile_work = nullptr;
-    int n_tile_work = 0;
-    // tiles grouped by LDS staging capacity: one launch per class
-    struct TileClass {
-        int scap;
-        int first;
-        int count;
-        int lane;  // 0: `stream`, k > 0: side[k-1]
-    };
-    std::vector<TileClass> classes;
-    int scap = 1024;  // largest class (single-system helpers)
-    bool mixed = false;
-    bool multi = false;
-    // captured step sequences, keyed by whether thermo is recorded
-    bool use_graph = false;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
-    // staging classes after the first run their list build and force on streams of their
-    // own (default priority), forked from and joined back into `stream` by events
-    std::vector<hipStream_t> side;
-    std::vector<hipEvent_t> join;
-    hipEvent_t fork = nullptr;
-    int nlanes = 1;
-    void* d_xfer = nullptr;
-    size_t xfer_bytes = 0;
-    // statistics / timing
-    uint64_t steps = 0, force_launches = 0;
-    bool timing = false;
-    std::vector<hipEvent_t> ev;
-    size_t ev_used = 0;
-    double force_ms = 0.0;
-    uint64_t timed_launches = 0;
-};
-
-static pis_status alloc_system(SysHost& h)
+Layout layout_of(const Plan& p)
 {
-    const Plan& p = h.plan;
     const size_t np = (size_t)p.npad, nc = (size_t)p.ncell;
+    Layout L{};
     size_t off = 0;
     auto take = [&](size_t bytes) {
         const size_t o = off;
         off += align256(bytes);
         return o;
     };
-    size_t o_st[2][11];
     for (int s = 0; s < 2; ++s) {
-        for (int k = 0; k < 9; ++k) o_st[s][k] = take(np * 8);
-        o_st[s][9] = take(np);
-        o_st[s][10] = take(np * 4);
+        for (int k = 0; k < 9; ++k) L.st[s][k] = take(np * 8);
+        L.st[s][9] = take(np);
+        L.st[s][10] = take(np * 4);
     }
-    const size_t o_r = take(np * 8 * 3);
-    const size_t o_ctl = take(64 * 4);
-    const size_t o_cc = take(nc * 4), o_cf = take(nc * 4), o_cs = take((nc + 1) * 4);
-    const size_t o_n2s = take(nc * 4), o_tc = take(((size_t)p.ntile + 1) * 4);
-    const size_t o_ac = take(np * 4), o_ss = take(np * 4), o_sk = take(np * 4);
-    const size_t o_nbr = take((size_t)(p.maxnb / 8) * np * 16);
-    const size_t o_nnb = take(np * 2);
-    const size_t o_part = take((size_t)p.ntile * 4 * 8);
-    const size_t o_pp = take(p.pp.size() * sizeof(PairParam));
-    const size_t o_m = take(p.mass.size() * 8);
+    L.r = take(np * 8 * 3);
+    L.ctl = take(64 * 4);
+    L.cc = take(nc * 4);
+    L.cf = take(nc * 4);
+    L.cs = take((nc + 1) * 4);
+    L.n2s = take(nc * 4);
+    L.tc = take(((size_t)p.ntile + 1) * 4);
+    L.ac = take(np * 4);
+    L.ss = take(np * 4);
+    L.sk = take(np * 4);
+    L.nbr = take((size_t)(p.maxnb / 8) * np * 16);
+    L.nnb = take(np * 2);
+    L.part = take((size_t)p.ntile * 4 * 8);
+    L.pp = take(p.pp.size() * sizeof(PairParam));
+    L.m = take(p.mass.size() * 8);
+    L.total = off;
+    return L;
+}
+
+}  // namespace
+
+size_t pis::system_bytes(const Plan& p, int thermo_records)
+{
+    return layout_of(p).total + align256(sizeof(ThermoDev) * (size_t)std::max(thermo_records, 0));
+}
+
+pis_status pis::alloc_system(SysHost& h)
+{
+    const Plan& p = h.plan;
+    const size_t np = (size_t)p.npad;
+    const Layout Lo = layout_of(p);
+    const size_t off = Lo.total;
+    const auto& o_st = Lo.st;
+    const size_t o_r = Lo.r, o_ctl = Lo.ctl, o_cc = Lo.cc, o_cf = Lo.cf, o_cs = Lo.cs, o_n2s = Lo.n2s,
+                 o_tc = Lo.tc, o_ac = Lo.ac, o_ss = Lo.ss, o_sk = Lo.sk, o_nbr = Lo.nbr, o_nnb = Lo.nnb,
+                 o_part = Lo.part, o_pp = Lo.pp, o_m = Lo.m;
     void* mem = nullptr;
     PIS_HIP(hipMalloc(&mem, off));
     h.mem = mem;
@@ -482,7 +443,7 @@ static pis_status alloc_system(SysHost& h)
     return PIS_OK;
 }
 
-static pis_status upload_sys(pis_ctx* c)
+pis_status pis::upload_sys(pis_ctx* c)
 {
     std::vector<SysDev> v(c->sys.size());
     for (size_t k = 0; k < c->sys.size(); ++k) v[k] = c->sys[k].dev;
@@ -491,7 +452,7 @@ static pis_status upload_sys(pis_ctx* c)
     return PIS_OK;
 }
 
-static void free_ctx(pis_ctx* c)
+void pis::free_ctx(pis_ctx* c)
 {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -515,7 +476,7 @@ static void free_ctx(pis_ctx* c)
 
 // Groups every tile of every system by its LDS staging class (one launch per class) and
 // uploads the work table.  Captured step graphs refer to the old table and are dropped.
-static pis_status group_tiles(pis_ctx* c)
+pis_status pis::group_tiles(pis_ctx* c)
 {
     std::map<int, std::vector<WorkItem>> by_class;
     c->scap = 1024;
@@ -573,6 +534,87 @@ static pis_status group_tiles(pis_ctx* c)
     return PIS_OK;
 }
 
+// Work tables and the device descriptor array after a membership change: the atom work
+// list, the tile classes (captured graphs dropped) and every SysDev.
+static pis_status relayout(pis_ctx* c)
+{
+    std::vector<WorkItem> aw;
+    c->multi = false;
+    for (size_t k = 0; k < c->sys.size(); ++k) {
+        const Plan& p = c->sys[k].plan;
+        c->multi |= p.ntypes > 1;
+        for (int64_t a = 0; a < p.n; a += kAtomThreads) aw.push_back(WorkItem{(int32_t)k, (int32_t)a});
+    }
+    if (c->d_atom_work) PIS_HIP(hipFree(c->d_atom_work));
+    c->d_atom_work = nullptr;
+    c->n_atom_work = (int)aw.size();
+    if (!aw.empty()) {
+        PIS_HIP(hipMalloc(&c->d_atom_work, sizeof(WorkItem) * aw.size()));
+        PIS_HIP(hipMemcpy(c->d_atom_work, aw.data(), sizeof(WorkItem) * aw.size(), hipMemcpyHostToDevice));
+    }
+    if (c->sys_cap < c->sys.size()) {
+        if (c->d_sys) PIS_HIP(hipFree(c->d_sys));
+        c->d_sys = nullptr;
+        const size_t cap = std::max<size_t>(c->sys.size(), 2 * c->sys_cap);
+        PIS_HIP(hipMalloc(&c->d_sys, sizeof(SysDev) * cap));
+        c->sys_cap = cap;
+    }
+    PIS_TRY(group_tiles(c));
+    if (!c->sys.empty()) PIS_TRY(upload_sys(c));
+    return PIS_OK;
+}
+
+pis_status pis::ctx_create_empty(bool mixed, pis_ctx** out)
+{
+    if (!out) return fail(PIS_ERR_INVALID_ARGUMENT, "null output pointer");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(PIS_ERR_NO_DEVICE, "no HIP device available");
+    pis_ctx* c = new pis_ctx();
+    c->device = 0;
+    c->mixed = mixed;
+    hipError_t e = hipSetDevice(c->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        free_ctx(c);
+        return fail(PIS_ERR_HIP, std::string("stream: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return PIS_OK;
+}
+
+pis_status pis::ctx_add(pis_ctx* c, Plan&& plan)
+{
+    if (plan.mixed != c->mixed)
+        return fail(PIS_ERR_INVALID_ARGUMENT, "all systems of a context need the same precision");
+    SysHost h;
+    h.plan = std::move(plan);
+    pis_status s = alloc_system(h);
+    if (s != PIS_OK) {
+        if (h.mem) (void)hipFree(h.mem);
+        return s;
+    }
+    c->sys.push_back(std::move(h));
+    return relayout(c);
+}
+
+pis_status pis::ctx_remove(pis_ctx* c, std::vector<size_t> idx)
+{
+    std::sort(idx.begin(), idx.end());
+    idx.erase(std::unique(idx.begin(), idx.end()), idx.end());
+    PIS_HIP(hipStreamSynchronize(c->stream));
+    for (size_t j = idx.size(); j-- > 0;) {
+        const size_t k = idx[j];
+        if (k >= c->sys.size()) return fail(PIS_ERR_INVALID_ARGUMENT, "system index out of range");
+        SysHost& h = c->sys[k];
+        if (h.mem) PIS_HIP(hipFree(h.mem));
+        if (h.thermo) PIS_HIP(hipFree(h.thermo));
+        c->sys.erase(c->sys.begin() + (ptrdiff_t)k);
+    }
+    return relayout(c);
+}
+
 static pis_status create_ctx(const pis_system_desc* descs, uint32_t nsys, pis_ctx** out)
 {
     if (!out) return fail(PIS_ERR_INVALID_ARGUMENT, "null output pointer");
@@ -585,20 +627,9 @@ static pis_status create_ctx(const pis_system_desc* descs, uint32_t nsys, pis_ct
         if (plans[k].mixed != plans[0].mixed)
             return fail(PIS_ERR_INVALID_ARGUMENT, "all systems of a batch need the same precision");
     }
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
-        return fail(PIS_ERR_NO_DEVICE, "no HIP device available");
-    pis_ctx* c = new pis_ctx();
-    c->device = 0;
-    hipError_t e = hipSetDevice(c->device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        free_ctx(c);
-        return fail(PIS_ERR_HIP, std::string("stream: ") + hipGetErrorString(e));
-    }
+    pis_ctx* c = nullptr;
+    PIS_TRY(ctx_create_empty(plans[0].mixed, &c));
     c->sys.resize(nsys);
-    c->mixed = plans[0].mixed;
-    std::vector<WorkItem> aw;
     for (uint32_t k = 0; k < nsys; ++k) {
         c->sys[k].plan = std::move(plans[k]);
         pis_status s = alloc_system(c->sys[k]);
@@ -606,25 +637,8 @@ static pis_status create_ctx(const pis_system_desc* descs, uint32_t nsys, pis_ct
             free_ctx(c);
             return s;
         }
-        const Plan& p = c->sys[k].plan;
-        c->multi |= p.ntypes > 1;
-        for (int64_t a = 0; a < p.n; a += kAtomThreads) aw.push_back(WorkItem{(int32_t)k, (int32_t)a});
     }
-    c->n_atom_work = (int)aw.size();
-    e = hipMalloc(&c->d_sys, sizeof(SysDev) * nsys);
-    if (e == hipSuccess) e = hipMalloc(&c->d_atom_work, sizeof(WorkItem) * aw.size());
-    if (e == hipSuccess) e = hipMemcpy(c->d_atom_work, aw.data(), sizeof(WorkItem) * aw.size(), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        free_ctx(c);
-        return fail(e == hipErrorOutOfMemory ? PIS_ERR_OUT_OF_MEMORY : PIS_ERR_HIP,
-                    std::string("context allocation: ") + hipGetErrorString(e));
-    }
-    pis_status st = group_tiles(c);
-    if (st != PIS_OK) {
-        free_ctx(c);
-        return st;
-    }
-    pis_status s = upload_sys(c);
+    pis_status s = relayout(c);
     if (s != PIS_OK) {
         free_ctx(c);
         return s;
@@ -633,7 +647,7 @@ static pis_status create_ctx(const pis_system_desc* descs, uint32_t nsys, pis_ct
     return PIS_OK;
 }
 
-static pis_status ensure_xfer(pis_ctx* c, size_t bytes)
+pis_status pis::ensure_xfer(pis_ctx* c, size_t bytes)
 {
     if (c->xfer_bytes >= bytes) return PIS_OK;
     if (c->d_xfer) PIS_HIP(hipFree(c->d_xfer));
@@ -644,14 +658,14 @@ static pis_status ensure_xfer(pis_ctx* c, size_t bytes)
     return PIS_OK;
 }
 
-static pis_status read_ctl(pis_ctx* c, uint32_t k, int32_t* ctl)
+pis_status pis::read_ctl(pis_ctx* c, uint32_t k, int32_t* ctl)
 {
     PIS_HIP(hipMemcpyAsync(ctl, c->sys[k].dev.ctl, CTL_COUNT * 4, hipMemcpyDeviceToHost, c->stream));
     PIS_HIP(hipStreamSynchronize(c->stream));
     return PIS_OK;
 }
 
-static pis_status write_ctl(pis_ctx* c, uint32_t k, const int32_t* ctl)
+pis_status pis::write_ctl(pis_ctx* c, uint32_t k, const int32_t* ctl)
 {
     PIS_HIP(hipMemcpyAsync(c->sys[k].dev.ctl, ctl, CTL_COUNT * 4, hipMemcpyHostToDevice, c->stream));
     PIS_HIP(hipStreamSynchronize(c->stream));
@@ -663,7 +677,7 @@ static pis_status write_ctl(pis_ctx* c, uint32_t k, const int32_t* ctl)
 // lane l > 0 run on side[l-1], forked after everything already on `stream` and joined
 // before anything enqueued later; captured into a graph, the fork becomes parallel
 // branches.
-static pis_status enqueue_classes(pis_ctx* c, bool build, int force)
+pis_status pis::enqueue_classes(pis_ctx* c, bool build, int force)
 {
     const int nl = c->nlanes;
     if (nl > 1) {
@@ -682,13 +696,13 @@ static pis_status enqueue_classes(pis_ctx* c, bool build, int force)
     return PIS_OK;
 }
 
-static pis_status enqueue_rebuild(pis_ctx* c)
+pis_status pis::enqueue_rebuild(pis_ctx* c)
 {
     launch_rebuild_sort(c->stream, c->d_sys, (int)c->sys.size(), c->d_atom_work, c->n_atom_work);
     return enqueue_classes(c, true, -1);
 }
 
-static pis_status run_rebuild(pis_ctx* c)
+pis_status pis::run_rebuild(pis_ctx* c)
 {
     PIS_TRY(enqueue_rebuild(c));
     launch_flip(c->stream, c->d_sys, (int)c->sys.size());
@@ -696,9 +710,10 @@ static pis_status run_rebuild(pis_ctx* c)
     return PIS_OK;
 }
 
-static pis_status check_flags(pis_ctx* c)
+pis_status pis::check_flags(pis_ctx* c, int only)
 {
     for (uint32_t k = 0; k < c->sys.size(); ++k) {
+        if (only >= 0 && (int)k != only) continue;
         int32_t ctl[CTL_COUNT];
         PIS_TRY(read_ctl(c, k, ctl));
         if (ctl[CTL_OVF_STAGE])
@@ -713,7 +728,7 @@ static pis_status check_flags(pis_ctx* c)
     return PIS_OK;
 }
 
-static pis_status sys_set_state(pis_ctx* c, uint32_t k, const double* pos, const double* vel,
+pis_status pis::sys_set_state(pis_ctx* c, uint32_t k, const double* pos, const double* vel,
                                 const double* force)
 {
     if (!c) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
@@ -762,10 +777,10 @@ static pis_status sys_set_state(pis_ctx* c, uint32_t k, const double* pos, const
         PIS_TRY(run_rebuild(c));
         PIS_HIP(hipStreamSynchronize(c->stream));
     }
-    return check_flags(c);
+    return check_flags(c, (int)k);
 }
 
-static pis_status sys_get_state(pis_ctx* c, uint32_t k, double* pos, double* vel, double* force)
+pis_status pis::sys_get_state(pis_ctx* c, uint32_t k, double* pos, double* vel, double* force)
 {
     if (!c) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
     if (k >= c->sys.size()) return fail(PIS_ERR_INVALID_ARGUMENT, "system index out of range");
@@ -783,7 +798,7 @@ static pis_status sys_get_state(pis_ctx* c, uint32_t k, double* pos, double* vel
     return PIS_OK;
 }
 
-static pis_status ensure_thermo(pis_ctx* c, int cap)
+pis_status pis::ensure_thermo(pis_ctx* c, int cap)
 {
     bool changed = false;
     for (auto& h : c->sys) {
@@ -861,7 +876,7 @@ static pis_status batch_compute(pis_ctx* c, pis_thermo* out /* [nsys] or null */
     PIS_TRY(ensure_thermo(c, 1));
     launch_set_ctl(c->stream, c->d_sys, (int)c->sys.size(), CTL_STEP, 0);
     PIS_TRY(launch_force_timed(c, 0));
-    launch_reduce(c->stream, c->d_sys, (int)c->sys.size(), 1, 1);
+    launch_reduce(c->stream, c->d_sys, (int)c->sys.size(), 1, 1, 0);
     PIS_HIP(hipGetLastError());
     PIS_HIP(hipStreamSynchronize(c->stream));
     PIS_TRY(collect_timing(c));
@@ -889,6 +904,7 @@ static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_
     PIS_TRY(upload_sys(c));
     const int ns = (int)c->sys.size();
     launch_set_ctl(c->stream, c->d_sys, ns, CTL_STEP, 0);
+    launch_set_ctl(c->stream, c->d_sys, ns, CTL_LEFT, -1);
     const int rec = out ? 1 : 0;
     if (c->use_graph && !c->timing) {
         // one captured step (every kernel's arguments are step-invariant: the thermo
@@ -899,7 +915,7 @@ static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_
             launch_integrate_pos(c->stream, c->d_sys, c->d_atom_work, c->n_atom_work);
             launch_rebuild_sort(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work);
             pis_status cs = enqueue_classes(c, true, 1);
-            launch_reduce(c->stream, c->d_sys, ns, 1, rec);
+            launch_reduce(c->stream, c->d_sys, ns, 1, rec, 1);
             hipError_t ce = hipStreamEndCapture(c->stream, &g);
             if (cs != PIS_OK) {
                 if (g) (void)hipGraphDestroy(g);
@@ -924,7 +940,7 @@ static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_
                 PIS_TRY(enqueue_classes(c, true, 1));
                 c->force_launches++;
             }
-            launch_reduce(c->stream, c->d_sys, ns, 1, rec);
+            launch_reduce(c->stream, c->d_sys, ns, 1, rec, 1);
             if ((t & 255) == 0) PIS_HIP(hipGetLastError());
         }
     }
@@ -961,6 +977,7 @@ const char* pis_status_string(pis_status s)
     case PIS_ERR_NO_DEVICE: return "no device";
     case PIS_ERR_CAPACITY: return "capacity exceeded";
     case PIS_ERR_NOT_FOUND: return "not found";
+    case PIS_ERR_TIMEOUT: return "timed out";
     }
     return "unknown status";
 }

@@ -20,7 +20,10 @@ void launch_build_list(hipStream_t st, const SysDev* d_sys, const WorkItem* tile
                        int scap);
 void launch_force(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work,
                   int n_tile_work, int scap, bool mixed, int integrate);
-void launch_reduce(hipStream_t st, const SysDev* d_sys, int nsys, int flip, int record);
+// `step` = 1 after an NVE step: frozen systems (ctl[CTL_LEFT] == 0) are skipped and a
+// positive step budget is counted down.
+void launch_reduce(hipStream_t st, const SysDev* d_sys, int nsys, int flip, int record, int step);
+void launch_gather_ctl(hipStream_t st, const SysDev* d_sys, int nsys, int32_t* out);
 void launch_flip(hipStream_t st, const SysDev* d_sys, int nsys);
 void launch_list_to_orig(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work,
                          int n_tile_work, int scap, uint32_t* out, uint32_t* counts);

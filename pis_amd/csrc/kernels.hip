@@ -315,7 +315,7 @@ __global__ __launch_bounds__(kAtomThreads) void k_integrate_pos(const SysDev* __
     const WorkItem w = work[blockIdx.x];
     const SysDev& s = sys[w.sys];
     const int a = w.idx + threadIdx.x;
-    if (a >= s.n) return;
+    if (a >= s.n || s.ctl[CTL_LEFT] == 0) return;
     const State st = s.st[s.ctl[CTL_CUR]];
     const double m = s.mass[st.type[a]];
     const double dt = s.dt, dt2 = s.dt2;
@@ -903,6 +903,7 @@ __global__ __launch_bounds__(kTileThreads, MIXED ? PIS_FORCE_MIN_WAVES_MIXED : P
 
     const WorkItem w = tiles[xcd_remap(blockIdx.x, ntw)];
     const SysDev& s = sys[w.sys];
+    if (integrate && s.ctl[CTL_LEFT] == 0) return;  // frozen system: the whole block leaves
     const State st = s.st[s.ctl[CTL_CUR] ^ (s.ctl[CTL_REBUILD] ? 1 : 0)];
     const TileGeom g = tile_geom(s, w.idx);
     const bool rel = s.mode == MODE_TILE_REL;
@@ -977,10 +978,12 @@ __global__ __launch_bounds__(kTileThreads, MIXED ? PIS_FORCE_MIN_WAVES_MIXED : P
 
 // Per-system fixed-order sum of the tile partials -> thermo record; optionally flips the
 // live buffer set after a rebuild.
-__global__ __launch_bounds__(1024) void k_reduce(const SysDev* __restrict__ sys, int flip, int record)
+__global__ __launch_bounds__(1024) void k_reduce(const SysDev* __restrict__ sys, int flip, int record,
+                                                 int step)
 {
     __shared__ double rtmp[64];
     const SysDev& s = sys[blockIdx.x];
+    if (step && s.ctl[CTL_LEFT] == 0) return;  // frozen: no step was taken
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     const double4* P4 = reinterpret_cast<const double4*>(s.partial);
     for (int t = threadIdx.x; t < s.ntile; t += blockDim.x) {
@@ -1014,6 +1017,15 @@ __global__ __launch_bounds__(1024) void k_reduce(const SysDev* __restrict__ sys,
         s.ctl[CTL_REBUILD] = 0;
         s.ctl[CTL_NREBUILD] += 1;
     }
+    if (step && s.ctl[CTL_LEFT] > 0) s.ctl[CTL_LEFT] -= 1;
+}
+
+// Every system's control words into one array [nsys][CTL_COUNT] (one copy to the host).
+__global__ void k_gather_ctl(const SysDev* __restrict__ sys, int nsys, int32_t* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nsys * CTL_COUNT) return;
+    out[i] = sys[i / CTL_COUNT].ctl[i % CTL_COUNT];
 }
 
 __global__ void k_flip(const SysDev* __restrict__ sys, int nsys)
@@ -1168,9 +1180,15 @@ void launch_force(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work
 #undef PIS_FORCE_D
 }
 
-void launch_reduce(hipStream_t st, const SysDev* d_sys, int nsys, int flip, int record)
+void launch_reduce(hipStream_t st, const SysDev* d_sys, int nsys, int flip, int record, int step)
 {
-    hipLaunchKernelGGL(k_reduce, dim3(nsys), dim3(1024), 0, st, d_sys, flip, record);
+    hipLaunchKernelGGL(k_reduce, dim3(nsys), dim3(1024), 0, st, d_sys, flip, record, step);
+}
+
+void launch_gather_ctl(hipStream_t st, const SysDev* d_sys, int nsys, int32_t* out)
+{
+    const int n = nsys * CTL_COUNT;
+    hipLaunchKernelGGL(k_gather_ctl, dim3((n + 255) / 256), dim3(256), 0, st, d_sys, nsys, out);
 }
 
 void launch_flip(hipStream_t st, const SysDev* d_sys, int nsys)

@@ -136,3 +136,13 @@ def test_argon_configuration_units():
     assert eps == pytest.approx(0.0104 * 96.4853)
     assert s.pairs[0][4] == pytest.approx(2.5 * 3.405)
     assert s.skin == pytest.approx(0.3 * 3.405)
+
+
+def test_server_needs_a_device_and_valid_arguments():
+    lib = pis_amd.lib()
+    srv = C.c_void_p()
+    assert lib.pis_server_create(1 << 30, 0, 10, 1, C.byref(srv)) == 5  # no slots
+    assert lib.pis_server_create(1 << 30, 4, 0, 1, C.byref(srv)) == 5   # no round length
+    assert lib.pis_server_create(1 << 30, 4, 10, 7, C.byref(srv)) == 5  # unknown precision
+    if lib.pis_device_count(C.byref(C.c_int32())) != 0:
+        assert lib.pis_server_create(1 << 30, 4, 10, 1, C.byref(srv)) == 9
