@@ -192,6 +192,25 @@ pis_status pis_batch_compute_potential(pis_ctx* ctx, double* pe_out);
 pis_status pis_batch_verlet_step_nve(pis_ctx* ctx, const double* dts, uint64_t nsteps,
                                      pis_thermo* thermo_out);
 
+/* ---- NVT: Nose-Hoover chain thermostat around the NVE step (potential.rs:35-58,
+ * ensemble/nvt.rs:4-119, the target ramp of simulation.rs:53-57).  The chain state lives
+ * on the device and is updated by the step kernels; velocities are scaled by
+ * exp(-dt xi0 / 2) before and after each NVE step.  The chain's kB is the system's kb. */
+typedef struct {
+    double t_start; /* fix nvt ... temp T_start T_end tau (NHThermostatChainArgs) */
+    double t_end;
+    double tau;
+    int32_t chain; /* chain length (the reference uses 3) */
+    int32_t reserved;
+} pis_nhc_params;
+/* NHThermostatChain::new_from_args for every system of the context (resets the chain). */
+pis_status pis_nvt_init(pis_ctx* ctx, const pis_nhc_params* params);
+/* nsteps of verlet_step_nvt_nhc with the target ramp over these nsteps; the per-step
+ * record's etot is the reference's NVT Hamiltonian (PE + KE + chain KE). */
+pis_status pis_verlet_step_nvt(pis_ctx* ctx, double dt, uint64_t nsteps, pis_thermo* per_step_out);
+pis_status pis_batch_verlet_step_nvt(pis_ctx* ctx, const double* dts, uint64_t nsteps,
+                                     pis_thermo* thermo_out);
+
 /* ---- Serving: continuous batching of NVE jobs on one GPU (no reference counterpart; the
  * reference runs one Simulation::run per process, simulation.rs:33-105).  A worker thread
  * keeps the admitted jobs' state resident and advances all of them together in rounds

@@ -558,6 +558,96 @@ double orc_pressure(const orc_sys* s, double ke)
 
 /* Simulation::run for NVE (simulation.rs:8-39, 41-52, 66-86): one initial compute with
  * forces starting from zero, then steps of verlet_step_nve with a thermo record each. */
+/* ---- Nose-Hoover chain (ensemble/nvt.rs) ------------------------------------ */
+
+int orc_nhc_new(orc_nhc* c, double t_start, double t_end, double tau, int32_t chain, double kb)
+{
+    if (chain < 1 || chain > ORC_NHC_MAX) return -1;
+    memset(c, 0, sizeof(*c));
+    c->chain = chain;
+    c->t_start = t_start;
+    c->t_end = t_end;
+    c->target = t_start; /* new_from_args passes start_temperature as the target */
+    c->kb = kb;
+    const double q_value = kb * c->target * (tau * tau); /* KB * T * tau.powi(2) */
+    double p10 = 1.0;
+    for (int i = 0; i < chain; ++i) {
+        c->q[i] = q_value / p10; /* 10.0_f64.powi(i): exact powers of ten */
+        p10 *= 10.0;
+    }
+    return 0;
+}
+
+void orc_nhc_compute_forces(orc_nhc* c, double ke, int64_t n)
+{
+    c->g[0] = 2.0 * ke - ((double)(n * 3)) * c->kb * c->target;
+    for (int j = 1; j < c->chain; ++j) c->g[j] = c->q[j - 1] * (c->xi[j - 1] * c->xi[j - 1]) - c->kb * c->target;
+}
+
+void orc_nhc_propagate_half_step(orc_nhc* c, double dt)
+{
+    const int j = c->chain - 1;
+    c->xi[j] = 0.5 * dt * c->g[j] / c->q[j];
+    for (int l = j - 1; l >= 0; --l) c->xi[l] = (0.5 * dt * c->g[l] / c->q[l]) * exp(-0.25 * dt * c->xi[l + 1]);
+}
+
+double orc_nhc_kinetic_energy(const orc_nhc* c)
+{
+    double k = 0.0;
+    for (int i = 0; i < c->chain; ++i) k += 0.5 * c->q[i] * (c->xi[i] * c->xi[i]);
+    return k;
+}
+
+double orc_nhc_potential_energy(const orc_nhc* c, int64_t n)
+{
+    double p = (double)(n * 3) * c->kb * c->target * c->eta[0];
+    for (int i = 1; i < c->chain; ++i) p += c->kb * c->target * c->eta[i];
+    return p;
+}
+
+void orc_nhc_target(orc_nhc* c, int64_t step, int64_t total)
+{
+    c->target = c->t_start + ((c->t_end - c->t_start) / (double)total) * (double)step;
+}
+
+double orc_verlet_step_nvt_nhc(orc_sys* s, double dt, orc_nhc* c, int32_t manager)
+{
+    double ke = orc_kinetic_energy(s);
+    orc_nhc_compute_forces(c, ke, s->n);
+    orc_nhc_propagate_half_step(c, dt);
+    const double scale = exp(-0.5 * dt * c->xi[0]);
+    for (int64_t k = 0; k < 3 * s->n; ++k) s->vel[k] = s->vel[k] * scale;
+    const double pe = orc_verlet_step_nve(s, dt, manager);
+    for (int64_t k = 0; k < 3 * s->n; ++k) s->vel[k] = s->vel[k] * scale;
+    ke = orc_kinetic_energy(s);
+    orc_nhc_compute_forces(c, ke, s->n);
+    orc_nhc_propagate_half_step(c, dt);
+    return pe;
+}
+
+int orc_run_nvt(orc_sys* s, double dt, int64_t nsteps, int32_t manager, double kb, double t_start,
+                double t_end, double tau, int32_t chain, double* thermo)
+{
+    orc_nhc c;
+    if (orc_nhc_new(&c, t_start, t_end, tau, chain, kb) != 0) return -2;
+    for (int64_t k = 0; k < 3 * s->n; ++k) s->force[k] = 0.0;
+    double pe0 = orc_compute(s, manager);
+    if (pe0 != pe0) return -1;
+    for (int64_t t = 0; t < nsteps; ++t) {
+        const double pe = orc_verlet_step_nvt_nhc(s, dt, &c, manager);
+        orc_nhc_target(&c, t, nsteps);
+        const double ke = orc_kinetic_energy(s);
+        if (thermo) {
+            thermo[5 * t + 0] = pe;
+            thermo[5 * t + 1] = ke;
+            thermo[5 * t + 2] = pe + ke + orc_nhc_kinetic_energy(&c) + orc_nhc_potential_energy(&c, s->n);
+            thermo[5 * t + 3] = orc_temperature(s, ke, kb);
+            thermo[5 * t + 4] = orc_pressure(s, ke);
+        }
+    }
+    return 0;
+}
+
 int orc_run_nve(orc_sys* s, double dt, int64_t nsteps, int32_t manager, double kb,
                 double* thermo)
 {

@@ -111,6 +111,30 @@ double orc_pressure(const orc_sys* s, double ke);
 int orc_run_nve(orc_sys* s, double dt, int64_t nsteps, int32_t manager, double kb,
                 double* thermo);
 
+/* Nose-Hoover chain (ensemble/nvt.rs:4-119), restated with its quirks: the half-step
+ * propagation *sets* xi from G (it does not add to it), eta never moves, and the chain
+ * masses keep the target temperature of construction. */
+#define ORC_NHC_MAX 8
+typedef struct {
+    int32_t chain;
+    double t_start, t_end, target, kb;
+    double xi[ORC_NHC_MAX], eta[ORC_NHC_MAX], g[ORC_NHC_MAX], q[ORC_NHC_MAX];
+} orc_nhc;
+
+/* NHThermostatChain::new via new_from_args (nvt.rs:22-57, 95-108): target = start. */
+int orc_nhc_new(orc_nhc* c, double t_start, double t_end, double tau, int32_t chain, double kb);
+void orc_nhc_compute_forces(orc_nhc* c, double ke, int64_t n);  /* nvt.rs:60-71 */
+void orc_nhc_propagate_half_step(orc_nhc* c, double dt);        /* nvt.rs:74-82 */
+double orc_nhc_kinetic_energy(const orc_nhc* c);                /* nvt.rs:84-90 */
+double orc_nhc_potential_energy(const orc_nhc* c, int64_t n);   /* nvt.rs:92-99 */
+void orc_nhc_target(orc_nhc* c, int64_t step, int64_t total);   /* nvt.rs:109-118 */
+/* verlet_step_nvt_nhc (potential.rs:35-58); returns PE(t+dt). */
+double orc_verlet_step_nvt_nhc(orc_sys* s, double dt, orc_nhc* c, int32_t manager);
+/* Simulation::run with Ensemble::NVT (simulation.rs:8-64, 90-116): per step writes PE,
+ * KE, H = PE + KE + chain KE + chain PE, T, P into thermo[5*k]. */
+int orc_run_nvt(orc_sys* s, double dt, int64_t nsteps, int32_t manager, double kb, double t_start,
+                double t_end, double tau, int32_t chain, double* thermo);
+
 /* Idealised parallel CPU baseline (not a restatement): full list + OpenMP gather. */
 int orc_ideal_run_nve(orc_sys* s, double dt, int64_t nsteps, double skin, int32_t nthreads,
                       double* pe_out);

@@ -45,6 +45,21 @@ class Sys(C.Structure):
     ]
 
 
+class NHC(C.Structure):
+    """orc_nhc: Nose-Hoover chain state (ensemble/nvt.rs:4-16)."""
+    _fields_ = [
+        ("chain", C.c_int32),
+        ("t_start", C.c_double),
+        ("t_end", C.c_double),
+        ("target", C.c_double),
+        ("kb", C.c_double),
+        ("xi", C.c_double * 8),
+        ("eta", C.c_double * 8),
+        ("g", C.c_double * 8),
+        ("q", C.c_double * 8),
+    ]
+
+
 class List(C.Structure):
     _fields_ = [
         ("n", C.c_int64),
@@ -88,6 +103,15 @@ def lib():
             "orc_pressure": (C.c_double, [C.POINTER(Sys), C.c_double]),
             "orc_run_nve": (C.c_int, [C.POINTER(Sys), C.c_double, C.c_int64, C.c_int32, C.c_double, _D]),
             "orc_ideal_run_nve": (C.c_int, [C.POINTER(Sys), C.c_double, C.c_int64, C.c_double, C.c_int32, _D]),
+            "orc_run_nvt": (C.c_int, [C.POINTER(Sys), C.c_double, C.c_int64, C.c_int32, C.c_double,
+                                      C.c_double, C.c_double, C.c_double, C.c_int32, _D]),
+            "orc_nhc_new": (C.c_int, [C.POINTER(NHC), C.c_double, C.c_double, C.c_double, C.c_int32, C.c_double]),
+            "orc_nhc_compute_forces": (None, [C.POINTER(NHC), C.c_double, C.c_int64]),
+            "orc_nhc_propagate_half_step": (None, [C.POINTER(NHC), C.c_double]),
+            "orc_nhc_kinetic_energy": (C.c_double, [C.POINTER(NHC)]),
+            "orc_nhc_potential_energy": (C.c_double, [C.POINTER(NHC), C.c_int64]),
+            "orc_nhc_target": (None, [C.POINTER(NHC), C.c_int64, C.c_int64]),
+            "orc_verlet_step_nvt_nhc": (C.c_double, [C.POINTER(Sys), C.c_double, C.POINTER(NHC), C.c_int32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -219,6 +243,14 @@ class OSystem:
         th = np.zeros((nsteps, 5))
         if lib().orc_run_nve(C.byref(self.s), dt, nsteps, manager, kb, th.ctypes.data_as(_D)) != 0:
             raise RuntimeError("oracle run failed")
+        return th
+
+    def run_nvt(self, dt, nsteps, t_start, t_end, tau, chain=3, manager=MGR_OFFSET, kb=1.0):
+        th = np.zeros((nsteps, 5))
+        rc = lib().orc_run_nvt(C.byref(self.s), dt, nsteps, manager, kb, t_start, t_end, tau, chain,
+                               th.ctypes.data_as(_D))
+        if rc != 0:
+            raise RuntimeError("oracle NVT run failed")
         return th
 
     def ideal_run_nve(self, dt, nsteps, skin, nthreads=0):

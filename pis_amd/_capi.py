@@ -102,6 +102,16 @@ class Stats(C.Structure):
     ]
 
 
+class NHCParams(C.Structure):
+    _fields_ = [
+        ("t_start", C.c_double),
+        ("t_end", C.c_double),
+        ("tau", C.c_double),
+        ("chain", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
 class JobResult(C.Structure):
     _fields_ = [
         ("job_id", C.c_uint64),
@@ -162,6 +172,9 @@ SIGNATURES = {
     "pis_sys_get_state": (C.c_int, [_P, C.c_uint32, _DP, _DP, _DP]),
     "pis_batch_compute_potential": (C.c_int, [_P, _DP]),
     "pis_batch_verlet_step_nve": (C.c_int, [_P, _DP, C.c_uint64, C.POINTER(Thermo)]),
+    "pis_nvt_init": (C.c_int, [_P, C.POINTER(NHCParams)]),
+    "pis_verlet_step_nvt": (C.c_int, [_P, C.c_double, C.c_uint64, C.POINTER(Thermo)]),
+    "pis_batch_verlet_step_nvt": (C.c_int, [_P, _DP, C.c_uint64, C.POINTER(Thermo)]),
     "pis_server_create": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(_P)]),
     "pis_server_submit": (
         C.c_int,

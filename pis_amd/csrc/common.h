@@ -58,6 +58,23 @@ struct PairParam {
     float l_lo32, l_hi32;
 };
 
+// Nose-Hoover chain state per system, in device memory (slots of SysDev::nhc).
+enum {
+    NHC_XI = 0,      // thermostat velocities xi[0..7]
+    NHC_Q = 8,       // masses q[0..7]
+    NHC_G = 16,      // forces G[0..7]
+    NHC_T = 24,      // target temperature
+    NHC_T0 = 25,     // start temperature
+    NHC_T1 = 26,     // end temperature
+    NHC_KB = 27,     // Boltzmann constant of the chain
+    NHC_SCALE = 28,  // exp(-dt xi[0] / 2) of the current step
+    NHC_STEP = 29,   // step index inside the current run (for the target ramp)
+    NHC_TOTAL = 30,  // steps of the current run
+    NHC_CHAIN = 31,  // chain length
+    NHC_COUNT = 40,
+    NHC_MAX_CHAIN = 8
+};
+
 struct State {
     double *x, *y, *z;
     double *vx, *vy, *vz;
@@ -99,7 +116,8 @@ struct SysDev {
     int32_t tile[3], nt[3], ntile;
     int32_t mode;
     int32_t thermo_cap;
-    int32_t pad0;
+    int32_t nhc_on;        // NVT: apply the chain's velocity scaling in the step kernels
+    double* nhc;           // [NHC_COUNT] Nose-Hoover chain state (device-updated)
 };
 
 // Work items of batched launches.

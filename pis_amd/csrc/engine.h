@@ -84,7 +84,8 @@ struct pis_ctx {
     bool multi = false;
     // captured step sequences, keyed by whether thermo is recorded
     bool use_graph = false;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};  // [record + 2 nvt]
+    bool nhc_ready = false;  // pis_nvt_init has set every system's chain
     // staging classes after the first run their list build and force on streams of their
     // own (default priority), forked from and joined back into `stream` by events
     std::vector<hipStream_t> side;

@@ -309,7 +309,7 @@ namespace {
 // Carving of one system's allocation (offsets of every array, 256-byte aligned).
 struct Layout {
     size_t st[2][11];
-    size_t r, ctl, cc, cf, cs, n2s, tc, ac, ss, sk, nbr, nnb, part, pp, m;
+    size_t r, ctl, cc, cf, cs, n2s, tc, ac, ss, sk, nbr, nnb, part, pp, m, nhc;
     size_t total;
 };
 
@@ -343,6 +343,7 @@ Layout layout_of(const Plan& p)
     L.part = take((size_t)p.ntile * 4 * 8);
     L.pp = take(p.pp.size() * sizeof(PairParam));
     L.m = take(p.mass.size() * 8);
+    L.nhc = take(NHC_COUNT * 8);
     L.total = off;
     return L;
 }
@@ -394,6 +395,8 @@ pis_status pis::alloc_system(SysHost& h)
     d.partial = (double*)(b + o_part);
     d.pp = (const PairParam*)(b + o_pp);
     d.mass = (const double*)(b + o_m);
+    d.nhc = (double*)(b + Lo.nhc);
+    d.nhc_on = 0;
     d.thermo = nullptr;
     d.thermo_cap = 0;
     for (int k = 0; k < 3; ++k) {
@@ -888,16 +891,20 @@ static pis_status batch_compute(pis_ctx* c, pis_thermo* out /* [nsys] or null */
     return PIS_OK;
 }
 
-static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_thermo* out)
+// nsteps of verlet_step_nve (nvt = false) or verlet_step_nvt_nhc (nvt = true, chain set
+// by pis_nvt_init) for every system, state resident on the device.
+static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_thermo* out, bool nvt = false)
 {
     if (!c) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
     if (!dts) return fail(PIS_ERR_INVALID_ARGUMENT, "null timestep array");
     if (nsteps == 0) return PIS_OK;
     if (nsteps > (uint64_t)1 << 30) return fail(PIS_ERR_INVALID_ARGUMENT, "too many steps");
+    if (nvt && !c->nhc_ready) return fail(PIS_ERR_INVALID_ARGUMENT, "NVT needs pis_nvt_init first");
     for (size_t k = 0; k < c->sys.size(); ++k) {
         if (!(dts[k] > 0.0) || !std::isfinite(dts[k])) return fail(PIS_ERR_INVALID_ARGUMENT, "dt must be > 0");
         c->sys[k].dev.dt = dts[k];
         c->sys[k].dev.dt2 = dts[k] * dts[k];  // dt.powi(2)
+        c->sys[k].dev.nhc_on = nvt ? 1 : 0;
     }
     const int cap = (int)std::min<uint64_t>(nsteps, (uint64_t)1 << 30);
     PIS_TRY(ensure_thermo(c, out ? cap : 1));
@@ -905,13 +912,17 @@ static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_
     const int ns = (int)c->sys.size();
     launch_set_ctl(c->stream, c->d_sys, ns, CTL_STEP, 0);
     launch_set_ctl(c->stream, c->d_sys, ns, CTL_LEFT, -1);
+    if (nvt) launch_nhc_run(c->stream, c->d_sys, ns, nsteps);
     const int rec = out ? 1 : 0;
+    const int gi = rec + (nvt ? 2 : 0);
     if (c->use_graph && !c->timing) {
         // one captured step (every kernel's arguments are step-invariant: the thermo
-        // record index and the live buffer set live on the device), replayed nsteps times
-        if (!c->graph[rec]) {
+        // record index, the live buffer set and the chain state live on the device),
+        // replayed nsteps times
+        if (!c->graph[gi]) {
             hipGraph_t g = nullptr;
             PIS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            if (nvt) launch_nhc_pre(c->stream, c->d_sys, ns);
             launch_integrate_pos(c->stream, c->d_sys, c->d_atom_work, c->n_atom_work);
             launch_rebuild_sort(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work);
             pis_status cs = enqueue_classes(c, true, 1);
@@ -922,14 +933,15 @@ static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_
                 return cs;
             }
             if (ce != hipSuccess) return fail(PIS_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
-            hipError_t e = hipGraphInstantiate(&c->graph[rec], g, nullptr, nullptr, 0);
+            hipError_t e = hipGraphInstantiate(&c->graph[gi], g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             if (e != hipSuccess) return fail(PIS_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
         }
-        for (uint64_t t = 0; t < nsteps; ++t) PIS_HIP(hipGraphLaunch(c->graph[rec], c->stream));
+        for (uint64_t t = 0; t < nsteps; ++t) PIS_HIP(hipGraphLaunch(c->graph[gi], c->stream));
         c->force_launches += nsteps;
     } else {
         for (uint64_t t = 0; t < nsteps; ++t) {
+            if (nvt) launch_nhc_pre(c->stream, c->d_sys, ns);
             launch_integrate_pos(c->stream, c->d_sys, c->d_atom_work, c->n_atom_work);
             launch_rebuild_sort(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work);
             if (c->timing) {
@@ -950,6 +962,38 @@ static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_
     PIS_TRY(collect_timing(c));
     PIS_TRY(check_flags(c));
     if (out) PIS_TRY(read_thermo(c, nsteps, out));
+    return PIS_OK;
+}
+
+// NHThermostatChain::new_from_args (nvt.rs:22-57, 95-108) for every system of the
+// context: xi = eta = G = 0, q_i = kB T_start tau^2 / 10^i, target = T_start.
+static pis_status nvt_init(pis_ctx* c, const pis_nhc_params* p)
+{
+    if (!c || !p) return fail(PIS_ERR_INVALID_ARGUMENT, "null argument");
+    if (p->chain < 1 || p->chain > NHC_MAX_CHAIN)
+        return fail(PIS_ERR_INVALID_ARGUMENT, "chain length must be 1.." + std::to_string(NHC_MAX_CHAIN));
+    if (!(p->tau > 0.0) || !std::isfinite(p->t_start) || !std::isfinite(p->t_end) || !(p->t_start > 0.0))
+        return fail(PIS_ERR_INVALID_ARGUMENT, "need tau > 0 and a positive start temperature");
+    for (auto& h : c->sys) {
+        double st[NHC_COUNT] = {0.0};
+        const double kb = h.plan.kb;
+        const double q_value = kb * p->t_start * (p->tau * p->tau);
+        double p10 = 1.0;
+        for (int i = 0; i < p->chain; ++i) {
+            st[NHC_Q + i] = q_value / p10;
+            p10 *= 10.0;
+        }
+        st[NHC_T] = p->t_start;
+        st[NHC_T0] = p->t_start;
+        st[NHC_T1] = p->t_end;
+        st[NHC_KB] = kb;
+        st[NHC_SCALE] = 1.0;
+        st[NHC_TOTAL] = 1.0;
+        st[NHC_CHAIN] = (double)p->chain;
+        PIS_HIP(hipMemcpyAsync(h.dev.nhc, st, sizeof(st), hipMemcpyHostToDevice, c->stream));
+    }
+    PIS_HIP(hipStreamSynchronize(c->stream));
+    c->nhc_ready = true;
     return PIS_OK;
 }
 
@@ -1083,6 +1127,20 @@ pis_status pis_verlet_step_nve(pis_ctx* ctx, double dt, uint64_t nsteps, pis_the
 pis_status pis_batch_verlet_step_nve(pis_ctx* ctx, const double* dts, uint64_t nsteps, pis_thermo* thermo_out)
 {
     return batch_run(ctx, dts, nsteps, thermo_out);
+}
+
+pis_status pis_nvt_init(pis_ctx* ctx, const pis_nhc_params* params) { return nvt_init(ctx, params); }
+
+pis_status pis_verlet_step_nvt(pis_ctx* ctx, double dt, uint64_t nsteps, pis_thermo* per_step_out)
+{
+    if (!ctx) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
+    if (ctx->sys.size() != 1) return fail(PIS_ERR_INVALID_ARGUMENT, "use pis_batch_verlet_step_nvt");
+    return batch_run(ctx, &dt, nsteps, per_step_out, true);
+}
+
+pis_status pis_batch_verlet_step_nvt(pis_ctx* ctx, const double* dts, uint64_t nsteps, pis_thermo* thermo_out)
+{
+    return batch_run(ctx, dts, nsteps, thermo_out, true);
 }
 
 pis_status pis_get_neighbor_list(pis_ctx* ctx, uint64_t* offsets, uint32_t* idx, uint64_t cap, uint64_t* total)

@@ -319,10 +319,21 @@ __global__ __launch_bounds__(kAtomThreads) void k_integrate_pos(const SysDev* __
     const State st = s.st[s.ctl[CTL_CUR]];
     const double m = s.mass[st.type[a]];
     const double dt = s.dt, dt2 = s.dt2;
+    double vx = st.vx[a], vy = st.vy[a], vz = st.vz[a];
+    if (s.nhc_on) {
+        // NVT: v *= exp(-dt xi0 / 2) before the NVE step (potential.rs:44-45)
+        const double sc = s.nhc[NHC_SCALE];
+        vx = vx * sc;
+        vy = vy * sc;
+        vz = vz * sc;
+        st.vx[a] = vx;
+        st.vy[a] = vy;
+        st.vz[a] = vz;
+    }
     // a_t = F/m (current_acceleration, properties.rs:32-39); x += v dt + (a 0.5) dt^2
-    double x = st.x[a] + (st.vx[a] * dt + (st.fx[a] / m) * 0.5 * dt2);
-    double y = st.y[a] + (st.vy[a] * dt + (st.fy[a] / m) * 0.5 * dt2);
-    double z = st.z[a] + (st.vz[a] * dt + (st.fz[a] / m) * 0.5 * dt2);
+    double x = st.x[a] + (vx * dt + (st.fx[a] / m) * 0.5 * dt2);
+    double y = st.y[a] + (vy * dt + (st.fy[a] / m) * 0.5 * dt2);
+    double z = st.z[a] + (vz * dt + (st.fz[a] / m) * 0.5 * dt2);
     // apply_boundary_conditions_pos (simulation_box.rs:29-42): s = h^-1 r; s -= floor(s); r = h s
     double sx = x * s.Linv[0], sy = y * s.Linv[1], sz = z * s.Linv[2];
     sx = sx - floor(sx);
@@ -951,6 +962,12 @@ __global__ __launch_bounds__(kTileThreads, MIXED ? PIS_FORCE_MIN_WAVES_MIXED : P
                 vx = vx + ((st.fx[a] / m + acc.fx / m) * 0.5) * dt;
                 vy = vy + ((st.fy[a] / m + acc.fy / m) * 0.5) * dt;
                 vz = vz + ((st.fz[a] / m + acc.fz / m) * 0.5) * dt;
+                if (s.nhc_on) {  // NVT: the second scaling (potential.rs:49)
+                    const double sc = s.nhc[NHC_SCALE];
+                    vx = vx * sc;
+                    vy = vy * sc;
+                    vz = vz * sc;
+                }
                 st.vx[a] = vx;
                 st.vy[a] = vy;
                 st.vz[a] = vz;
@@ -978,13 +995,34 @@ __global__ __launch_bounds__(kTileThreads, MIXED ? PIS_FORCE_MIN_WAVES_MIXED : P
 
 // Per-system fixed-order sum of the tile partials -> thermo record; optionally flips the
 // live buffer set after a rebuild.
-__global__ __launch_bounds__(1024) void k_reduce(const SysDev* __restrict__ sys, int flip, int record,
-                                                 int step)
+// Nose-Hoover chain arithmetic (ensemble/nvt.rs:59-90), one thread, op for op.
+__device__ void nhc_forces(double* c, double ke, int64_t n)
 {
-    __shared__ double rtmp[64];
-    const SysDev& s = sys[blockIdx.x];
-    if (step && s.ctl[CTL_LEFT] == 0) return;  // frozen: no step was taken
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    const int nc = (int)c[NHC_CHAIN];
+    const double kb = c[NHC_KB], T = c[NHC_T];
+    c[NHC_G] = 2.0 * ke - ((double)(n * 3)) * kb * T;
+    for (int j = 1; j < nc; ++j) c[NHC_G + j] = c[NHC_Q + j - 1] * (c[NHC_XI + j - 1] * c[NHC_XI + j - 1]) - kb * T;
+}
+
+__device__ void nhc_half_step(double* c, double dt)
+{
+    const int j = (int)c[NHC_CHAIN] - 1;
+    c[NHC_XI + j] = 0.5 * dt * c[NHC_G + j] / c[NHC_Q + j];
+    for (int l = j - 1; l >= 0; --l)
+        c[NHC_XI + l] = (0.5 * dt * c[NHC_G + l] / c[NHC_Q + l]) * exp(-0.25 * dt * c[NHC_XI + l + 1]);
+}
+
+__device__ double nhc_kinetic(const double* c)
+{
+    double k = 0.0;
+    for (int i = 0; i < (int)c[NHC_CHAIN]; ++i) k += 0.5 * c[NHC_Q + i] * (c[NHC_XI + i] * c[NHC_XI + i]);
+    return k;
+}
+
+// Sum of a system's per-tile partials (fixed order; identical to k_reduce's sums).
+__device__ __forceinline__ void sum_partials(const SysDev& s, double* v, double* rtmp)
+{
+    v[0] = v[1] = v[2] = v[3] = 0.0;
     const double4* P4 = reinterpret_cast<const double4*>(s.partial);
     for (int t = threadIdx.x; t < s.ntile; t += blockDim.x) {
         const double4 q = P4[t];
@@ -994,15 +1032,62 @@ __global__ __launch_bounds__(1024) void k_reduce(const SysDev* __restrict__ sys,
         v[3] += q.w;
     }
     block_sum4(v, rtmp);
+}
+
+// Start of an NVT step (potential.rs:41-45): KE of the current velocities (the last
+// force launch's partials), chain forces and half step, and the velocity scale.
+__global__ __launch_bounds__(1024) void k_nhc_pre(const SysDev* __restrict__ sys)
+{
+    __shared__ double rtmp[64];
+    const SysDev& s = sys[blockIdx.x];
+    if (!s.nhc_on || s.ctl[CTL_LEFT] == 0) return;
+    double v[4];
+    sum_partials(s, v, rtmp);
+    if (threadIdx.x != 0) return;
+    double* c = s.nhc;
+    nhc_forces(c, v[1], s.n);
+    nhc_half_step(c, s.dt);
+    c[NHC_SCALE] = exp(-0.5 * s.dt * c[NHC_XI]);
+}
+
+// Step counter and length of an NVT run (the target ramp's i and total).
+__global__ void k_nhc_run(const SysDev* __restrict__ sys, int nsys, double total)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nsys || !sys[k].nhc_on) return;
+    sys[k].nhc[NHC_STEP] = 0.0;
+    sys[k].nhc[NHC_TOTAL] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_reduce(const SysDev* __restrict__ sys, int flip, int record,
+                                                 int step)
+{
+    __shared__ double rtmp[64];
+    const SysDev& s = sys[blockIdx.x];
+    if (step && s.ctl[CTL_LEFT] == 0) return;  // frozen: no step was taken
+    double v[4];
+    sum_partials(s, v, rtmp);
     if (threadIdx.x != 0) return;
     const double pe = 0.5 * v[0], ke = v[1], wabs = v[2], wpair = 0.5 * v[3];
+    double ham = pe + ke;
+    if (step && s.nhc_on) {
+        // end of an NVT step (potential.rs:51-54), H with the chain's energy
+        // (simulation.rs:100-105; the chain coordinates eta never move, so its potential
+        // energy is 0), then the target ramp (simulation.rs:55-56, nvt.rs:109-118)
+        double* c = s.nhc;
+        nhc_forces(c, ke, s.n);
+        nhc_half_step(c, s.dt);
+        ham = ham + nhc_kinetic(c) + 0.0;
+        c[NHC_T] = c[NHC_T0] + ((c[NHC_T1] - c[NHC_T0]) / c[NHC_TOTAL]) * c[NHC_STEP];
+        c[NHC_STEP] = c[NHC_STEP] + 1.0;
+    }
     if (record) {
         const int k = s.ctl[CTL_STEP];
         if (k < s.thermo_cap) {
             ThermoDev t;
             t.pe = pe;
             t.ke = ke;
-            t.etot = pe + ke;
+            t.etot = ham;
             t.temperature = (2.0 * ke) / ((double)(3 * (int64_t)s.n) * s.kb);
             t.pressure = (2.0 * ke + wabs) / (3.0 * s.volume);
             t.virial_abs = wabs;
@@ -1183,6 +1268,16 @@ void launch_force(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work
 void launch_reduce(hipStream_t st, const SysDev* d_sys, int nsys, int flip, int record, int step)
 {
     hipLaunchKernelGGL(k_reduce, dim3(nsys), dim3(1024), 0, st, d_sys, flip, record, step);
+}
+
+void launch_nhc_pre(hipStream_t st, const SysDev* d_sys, int nsys)
+{
+    hipLaunchKernelGGL(k_nhc_pre, dim3(nsys), dim3(1024), 0, st, d_sys);
+}
+
+void launch_nhc_run(hipStream_t st, const SysDev* d_sys, int nsys, uint64_t total)
+{
+    hipLaunchKernelGGL(k_nhc_run, dim3((nsys + 255) / 256), dim3(256), 0, st, d_sys, nsys, (double)total);
 }
 
 void launch_gather_ctl(hipStream_t st, const SysDev* d_sys, int nsys, int32_t* out)

@@ -148,6 +148,21 @@ class Engine:
             return None
         return thermo_to_array(recs).reshape(nsteps, self.nsys, 7)
 
+    def nvt_init(self, t_start: float, t_end: float, tau: float, chain: int = 3):
+        """Nose-Hoover chain for every system (NHThermostatChain::new_from_args)."""
+        prm = _capi.NHCParams(float(t_start), float(t_end), float(tau), int(chain), 0)
+        check(lib().pis_nvt_init(self._ptr, C.byref(prm)))
+
+    def run_nvt(self, dt, nsteps: int, record: bool = True):
+        """nsteps of verlet_step_nvt_nhc for every system; thermo [nsteps, nsys, 7] with
+        etot = PE + KE + chain KE."""
+        dts = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (self.nsys,)))
+        recs = (_capi.Thermo * (nsteps * self.nsys))() if record else None
+        check(lib().pis_batch_verlet_step_nvt(self._ptr, _dptr(dts), nsteps, recs))
+        if not record:
+            return None
+        return thermo_to_array(recs).reshape(nsteps, self.nsys, 7)
+
     def neighbor_list(self):
         """(offsets, idx) CSR of the full list at the current positions, original indices."""
         n = self.specs[0].n

@@ -111,3 +111,26 @@ def test_dense_halo_grows_staging_class():
     assert force_err(f, o.force) <= 1e-10
     assert abs(pe - pe_ref) <= 1e-10 * abs(pe_ref)
     assert np.all(np.isfinite(th))
+
+
+def test_nvt_batch_equals_alone_and_graph_equals_direct():
+    specs = small_ensemble()[:5]
+    nsteps = 120
+
+    def run_nvt(ss, graph):
+        with Engine(ss, "mixed") as eng:
+            eng.set_graph_mode(graph)
+            eng.load()
+            eng.compute_potential()
+            eng.nvt_init(1.1, 0.9, 0.3, 3)
+            th = eng.run_nvt([s.dt for s in ss], nsteps)
+            return th, [eng.get_state(k) for k in range(len(ss))]
+
+    th_b, st_b = run_nvt(specs, False)
+    th_g, st_g = run_nvt(specs, True)
+    assert np.array_equal(th_b, th_g)
+    for k, s in enumerate(specs):
+        th_a, st_a = run_nvt([s], False)
+        assert np.array_equal(th_a[:, 0, :], th_b[:, k, :])
+        for a, b, c in zip(st_a[0], st_b[k], st_g[k]):
+            assert np.array_equal(a, b) and np.array_equal(b, c)

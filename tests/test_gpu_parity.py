@@ -163,3 +163,28 @@ def test_errors_map_to_status():
         with pytest.raises(PisError) as e:
             eng.compute_potential()
         assert e.value.status == 3
+
+
+def test_nvt_tracks_oracle_nose_hoover_chain():
+    """verlet_step_nvt_nhc with the target ramp (potential.rs:35-58, simulation.rs:53-57):
+    FP64 GPU thermo against the oracle over 400 steps of the argon crystal heated from
+    100 K toward 140 K (tau = 100 fs); etot is the reference's NVT Hamiltonian (PE + KE +
+    chain KE)."""
+    spec = W.argon_fcc_c1()
+    nsteps = 400
+    with Engine(spec, "fp64") as eng:
+        eng.load()
+        eng.compute_potential()
+        eng.nvt_init(100.0, 140.0, 1.0, 3)
+        th = eng.run_nvt(spec.dt, nsteps)[:, 0, :]
+        pos, vel, _ = eng.get_state(0)
+    o = po.OSystem.from_spec(spec)
+    ref = o.run_nvt(spec.dt, nsteps, 100.0, 140.0, 1.0, 3, po.MGR_OFFSET, kb=spec.kb)
+    scale = np.abs(ref[:, 0]).max()
+    for col in (0, 1, 2):
+        assert np.abs(th[:, col] - ref[:, col]).max() <= 1e-9 * scale
+    assert np.abs(th[:, 3] - ref[:, 3]).max() <= 1e-7 * np.abs(ref[:, 3]).max()
+    assert ref[-100:, 3].mean() > ref[:100, 3].mean()  # the ramp heated it
+    L = spec.box.lengths
+    d = (pos - o.pos + L / 2) % L - L / 2
+    assert np.abs(d).max() <= 1e-8

@@ -444,3 +444,75 @@ def test_run_matches_step_by_step_and_reports_thermo():
         ke = b.kinetic_energy()
         assert th[t, 0] == pe and th[t, 1] == ke and th[t, 2] == pe + ke
     np.testing.assert_array_equal(a.pos, b.pos)
+
+
+# ---------------------------------------------------------------------- NVT chain
+# ensemble/nvt.rs:4-119, potential.rs:35-58, simulation.rs:53-57,100-105
+
+
+def _nhc(t_start, t_end, tau, chain=3, kb=1.0):
+    import ctypes as C
+
+    c = po.NHC()
+    assert po.lib().orc_nhc_new(C.byref(c), t_start, t_end, tau, chain, kb) == 0
+    return c
+
+
+def test_chain_masses_follow_the_relaxation_time():
+    # Q_1 = N_f-independent kB T tau^2 of the chain paper, later links 10x lighter each
+    c = _nhc(1.3, 1.3, 0.4, chain=3, kb=0.5)
+    q0 = 0.5 * 1.3 * 0.16
+    assert list(c.q)[:3] == pytest.approx([q0, q0 / 10.0, q0 / 100.0], rel=1e-15)
+    assert c.target == 1.3 and list(c.xi)[:3] == [0.0, 0.0, 0.0]
+
+
+def test_chain_force_vanishes_at_the_target_kinetic_energy():
+    import ctypes as C
+
+    n, T = 500, 1.7
+    c = _nhc(T, T, 0.5)
+    po.lib().orc_nhc_compute_forces(C.byref(c), 1.5 * n * T, n)  # KE = (3N/2) kB T
+    assert c.g[0] == pytest.approx(0.0, abs=1e-12)
+    # idle links feel -kB T (their upstream velocity is zero)
+    assert c.g[1] == -T and c.g[2] == -T
+
+
+def test_chain_cools_a_hot_system_and_heats_a_cold_one():
+    import ctypes as C
+
+    n = 100
+    for ke, sign in ((1.5 * n * 2.0, 1.0), (1.5 * n * 0.5, -1.0)):
+        c = _nhc(1.0, 1.0, 0.5)
+        po.lib().orc_nhc_compute_forces(C.byref(c), ke, n)
+        po.lib().orc_nhc_propagate_half_step(C.byref(c), 0.005)
+        # xi_0 > 0 scales velocities down (exp(-dt xi/2) < 1), xi_0 < 0 scales them up
+        assert np.sign(c.xi[0]) == sign
+
+
+def test_target_ramp_is_linear_in_the_finished_step():
+    import ctypes as C
+
+    c = _nhc(1.0, 2.0, 0.5)
+    for i, want in ((0, 1.0), (5, 1.5), (9, 1.9)):
+        po.lib().orc_nhc_target(C.byref(c), i, 10)
+        assert c.target == pytest.approx(want, rel=1e-15)
+
+
+@pytest.mark.parametrize("t_target", [1.4, 0.5])
+def test_nvt_run_relaxes_to_the_target_temperature(t_target):
+    spec = W.lj_liquid(6, 0.8442, 0.9, seed=31)
+    o = po.OSystem.from_spec(spec)
+    th = o.run_nvt(0.004, 1500, t_target, t_target, 0.2, 3, kb=1.0)
+    t_late = th[-500:, 3].mean()
+    assert abs(t_late - t_target) < 0.06 * t_target
+
+
+def test_nvt_with_a_stiff_chain_reduces_to_nve():
+    # tau -> infinity: Q huge, xi -> 0, both scalings -> 1
+    spec = W.lj_liquid(6, 0.8442, 0.72, seed=32)
+    a = po.OSystem.from_spec(spec)
+    b = po.OSystem.from_spec(spec)
+    th_nvt = a.run_nvt(0.005, 50, 0.72, 0.72, 1e9, 3, kb=1.0)
+    th_nve = b.run_nve(0.005, 50, po.MGR_OFFSET, kb=1.0)
+    assert np.abs(th_nvt[:, 0] - th_nve[:, 0]).max() <= 1e-9 * np.abs(th_nve[:, 0]).max()
+    assert np.abs(a.pos - b.pos).max() <= 1e-12
