@@ -60,8 +60,7 @@ def force_kernel_bytes_per_atom(nbar):
     """Algorithmic HBM bytes of one force launch per atom: the u16 list entries
     (2 B each) and their count (2 B), position (24 B) and type (1 B) of the atom, old
     force and velocity read (48 B), new force and velocity written (48 B).  Neighbour
-    positions are staged once per tile from L2/MALL and counted at zero, like the
-    SURVEY's step model does."""
+    positions are staged once per tile from L2/MALL and counted at zero."""
     return 2.0 * nbar + 2 + 24 + 1 + 48 + 48
 
 
@@ -123,6 +122,26 @@ def run_gpu(spec, precision, steps, warmup):
         "thermo": th,
         "f0": f0,
     }
+
+
+def run_per_call(spec, steps):
+    """The drop-in per-call path (LJGpuManager.verlet_step_nve, the reference's call
+    pattern): every step ships positions, velocities and forces to the device and back
+    over PCIe.  Reported beside the device-resident rate, never as the headline."""
+    from pis_amd import Atoms, LennardJones, LJGpuManager
+
+    mgr = LJGpuManager(skin=spec.skin, kb=spec.kb, precision="mixed")
+    for (ti, tj, eps, sig, rc) in spec.pairs:
+        mgr.insert((ti, tj), LennardJones(eps, sig, rc, True))
+    atoms = Atoms(spec.type_ids, spec.masses, spec.positions, spec.velocities, spec.box)
+    mgr.compute_potential(atoms)
+    mgr.verlet_step_nve(atoms, spec.dt)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        mgr.verlet_step_nve(atoms, spec.dt)
+    sec = time.perf_counter() - t0
+    return {"value": round(spec.n * steps / sec, 1), "unit": "atom-steps/s", "steps": steps,
+            "ms_per_step": round(1e3 * sec / steps, 3)}
 
 
 def run_ensemble(steps, warmup, direct_steps):
@@ -277,6 +296,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle force check")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--per-call-steps", type=int, default=20,
+                    help="steps of the PCIe-inclusive per-call path (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=5000,
                     help="steps of the 512-system batched ensemble (0: skip)")
     ap.add_argument("--serve-jobs", type=int, default=100,
@@ -285,8 +306,8 @@ def main():
                     help="bytes of the streaming-copy calibration run after the timed runs (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(the multi-GPU design is described in DESIGN.md, nothing here launches it).",
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(systems shard by spatial domain or by job across GPUs; nothing here launches that).",
               file=sys.stderr)
         return 2
 
@@ -304,6 +325,12 @@ def main():
         fp64 = run_gpu(spec, "fp64", args.steps, args.warmup)
         log(f"fp64: {fp64['rate']:.4e} atom-steps/s, {fp64['ms_per_step']:.4f} ms/step, "
             f"force kernel {fp64['force_ms']:.4f} ms")
+
+    per_call = None
+    if args.per_call_steps:
+        log(f"per-call path (state over PCIe every step): {args.per_call_steps} steps")
+        per_call = run_per_call(spec, args.per_call_steps)
+        log(f"per-call: {per_call['value']:.4e} atom-steps/s")
 
     ensemble = None
     if args.ensemble_steps:
@@ -410,6 +437,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched_ensemble": ensemble,
+        "pcie_inclusive_per_call": per_call,
         "job_latency": serving,
     }
     res.update(out_extra)

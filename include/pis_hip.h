@@ -210,6 +210,10 @@ pis_status pis_nvt_init(pis_ctx* ctx, const pis_nhc_params* params);
 pis_status pis_verlet_step_nvt(pis_ctx* ctx, double dt, uint64_t nsteps, pis_thermo* per_step_out);
 pis_status pis_batch_verlet_step_nvt(pis_ctx* ctx, const double* dts, uint64_t nsteps,
                                      pis_thermo* thermo_out);
+/* The chain's thermostat velocities xi[chain] and current target temperature of system
+ * `sys` (NHThermostatChain::xi / target_temperature, nvt.rs:9-13), for callers that keep a
+ * host copy of the chain (either pointer may be NULL). */
+pis_status pis_nvt_get_chain(pis_ctx* ctx, uint32_t sys, double* xi, double* target);
 
 /* ---- Serving: continuous batching of NVE jobs on one GPU (no reference counterpart; the
  * reference runs one Simulation::run per process, simulation.rs:33-105).  A worker thread

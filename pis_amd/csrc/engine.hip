@@ -1131,6 +1131,20 @@ pis_status pis_batch_verlet_step_nve(pis_ctx* ctx, const double* dts, uint64_t n
 
 pis_status pis_nvt_init(pis_ctx* ctx, const pis_nhc_params* params) { return nvt_init(ctx, params); }
 
+pis_status pis_nvt_get_chain(pis_ctx* ctx, uint32_t sys, double* xi, double* target)
+{
+    if (!ctx) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
+    if (sys >= ctx->sys.size()) return fail(PIS_ERR_INVALID_ARGUMENT, "system index out of range");
+    if (!ctx->nhc_ready) return fail(PIS_ERR_INVALID_ARGUMENT, "NVT needs pis_nvt_init first");
+    double st[NHC_COUNT];
+    PIS_HIP(hipMemcpyAsync(st, ctx->sys[sys].dev.nhc, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+    PIS_HIP(hipStreamSynchronize(ctx->stream));
+    if (xi)
+        for (int i = 0; i < (int)st[NHC_CHAIN]; ++i) xi[i] = st[NHC_XI + i];
+    if (target) *target = st[NHC_T];
+    return PIS_OK;
+}
+
 pis_status pis_verlet_step_nvt(pis_ctx* ctx, double dt, uint64_t nsteps, pis_thermo* per_step_out)
 {
     if (!ctx) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");

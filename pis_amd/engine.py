@@ -163,6 +163,13 @@ class Engine:
             return None
         return thermo_to_array(recs).reshape(nsteps, self.nsys, 7)
 
+    def nvt_chain(self, k: int = 0, chain: int = 3):
+        """(xi[chain], target temperature) of system k's thermostat chain."""
+        xi = np.zeros(8)
+        t = C.c_double(0.0)
+        check(lib().pis_nvt_get_chain(self._ptr, k, _dptr(xi), C.byref(t)))
+        return xi[:chain].copy(), t.value
+
     def neighbor_list(self):
         """(offsets, idx) CSR of the full list at the current positions, original indices."""
         n = self.specs[0].n

@@ -178,6 +178,7 @@ def test_nvt_tracks_oracle_nose_hoover_chain():
         eng.nvt_init(100.0, 140.0, 1.0, 3)
         th = eng.run_nvt(spec.dt, nsteps)[:, 0, :]
         pos, vel, _ = eng.get_state(0)
+        xi, target = eng.nvt_chain(0)
     o = po.OSystem.from_spec(spec)
     ref = o.run_nvt(spec.dt, nsteps, 100.0, 140.0, 1.0, 3, po.MGR_OFFSET, kb=spec.kb)
     scale = np.abs(ref[:, 0]).max()
@@ -185,6 +186,9 @@ def test_nvt_tracks_oracle_nose_hoover_chain():
         assert np.abs(th[:, col] - ref[:, col]).max() <= 1e-9 * scale
     assert np.abs(th[:, 3] - ref[:, 3]).max() <= 1e-7 * np.abs(ref[:, 3]).max()
     assert ref[-100:, 3].mean() > ref[:100, 3].mean()  # the ramp heated it
+    # after the last step the ramp stands at step nsteps-1 (nvt.rs:109-118)
+    assert target == pytest.approx(100.0 + 40.0 / nsteps * (nsteps - 1), rel=1e-15)
+    assert np.all(np.isfinite(xi)) and xi[0] != 0.0
     L = spec.box.lengths
     d = (pos - o.pos + L / 2) % L - L / 2
     assert np.abs(d).max() <= 1e-8
