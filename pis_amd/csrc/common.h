@@ -98,7 +98,8 @@ struct SysDev {
     int32_t* tile_cell0;   // [ntile+1]
     int32_t* atom_cell;    // [n] sorted cell id of atom (live order) at binning
     int32_t* scat_src;     // [n] scatter slot -> source atom
-    uint32_t* scat_key;    // [n] scatter slot -> orig index (sort key)
+    uint32_t* scat_key;    // [n] scatter slot -> orig index (sort key, ties)
+    double* scat_x;        // [n] scatter slot -> x (sort key)
     uint4* nbr;            // [(maxnb/8) * npad]
     uint16_t* nnb;         // [npad]
     double* partial;       // [ntile*4]: pe, ke, virial_abs, virial_pair

@@ -309,7 +309,7 @@ namespace {
 // Carving of one system's allocation (offsets of every array, 256-byte aligned).
 struct Layout {
     size_t st[2][11];
-    size_t r, ctl, cc, cf, cs, n2s, tc, ac, ss, sk, nbr, nnb, part, pp, m, nhc;
+    size_t r, ctl, cc, cf, cs, n2s, tc, ac, ss, sk, sx, nbr, nnb, part, pp, m, nhc;
     size_t total;
 };
 
@@ -338,6 +338,7 @@ Layout layout_of(const Plan& p)
     L.ac = take(np * 4);
     L.ss = take(np * 4);
     L.sk = take(np * 4);
+    L.sx = take(np * 8);
     L.nbr = take((size_t)(p.maxnb / 8) * np * 16);
     L.nnb = take(np * 2);
     L.part = take((size_t)p.ntile * 4 * 8);
@@ -390,6 +391,7 @@ pis_status pis::alloc_system(SysHost& h)
     d.atom_cell = (int32_t*)(b + o_ac);
     d.scat_src = (int32_t*)(b + o_ss);
     d.scat_key = (uint32_t*)(b + o_sk);
+    d.scat_x = (double*)(b + Lo.sx);
     d.nbr = (uint4*)(b + o_nbr);
     d.nnb = (uint16_t*)(b + o_nnb);
     d.partial = (double*)(b + o_part);

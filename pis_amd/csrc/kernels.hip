@@ -426,6 +426,7 @@ __global__ __launch_bounds__(kAtomThreads) void k_scatter(const SysDev* __restri
         const int p = s.cell_start[sc] + atomicAdd(&s.cell_fill[sc], 1);
         s.scat_src[p] = a;
         s.scat_key[p] = st.orig[a];
+        s.scat_x[p] = st.x[a];
     }
 }
 
@@ -451,7 +452,7 @@ __global__ __launch_bounds__(kAtomThreads) void k_rank_permute(const SysDev* __r
         const double x = src.x[a], y = src.y[a], z = src.z[a];
         int rank = 0;
         for (int q = c0; q < c1; ++q) {
-            const double xq = src.x[s.scat_src[q]];
+            const double xq = s.scat_x[q];
             const uint32_t kq = s.scat_key[q];
             rank += (xq < x || (xq == x && kq < key)) ? 1 : 0;
         }
@@ -573,7 +574,7 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
     const double ylo = (ly - 0.5 * g.sy) * s.cellw[1], zlo = (lz - 0.5 * g.sz) * s.cellw[2];
     const double gym = fmax(0.0, (qy - ylo) - slack), gyp = fmax(0.0, (ylo + s.cellw[1] - qy) - slack);
     const double gzm = fmax(0.0, (qz - zlo) - slack), gzp = fmax(0.0, (zlo + s.cellw[2] - qz) - slack);
-    uint64_t lo = 0, hi = 0;
+    uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;  // 8 list entries of 16 bits, w0 oldest
     int cnt = 0;
     const int maxg = s.maxnb >> 3;
     uint4* out = s.nbr + a;
@@ -607,28 +608,25 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
         // uniform over the wave by construction; readfirstlane makes the loops scalar
         const int rowmax = __builtin_amdgcn_readfirstlane(wave_max(re - rb));
         for (int cb = 0; cb < rowmax; cb += 64) {
-            const int kmax = min(64, rowmax - cb);
-            uint32_t mlo = 0, mhi = 0;
-            bool band = false;
-            // PIS_BUILD_UNROLL candidates per trip: their LDS loads issue together and
-            // their distance chains interleave, which is what hides latency in a tile
-            // whose few waves cannot hide it for each other
-            for (int k = 0; k < kmax; k += PIS_BUILD_UNROLL) {
-                float dx[PIS_BUILD_UNROLL], dy[PIS_BUILD_UNROLL], dz[PIS_BUILD_UNROLL];
-                int tt[PIS_BUILD_UNROLL];
-                bool ok[PIS_BUILD_UNROLL];
+            constexpr int U = PIS_BUILD_UNROLL;
+            // trips of U candidates up to the wave's longest window (<= 64); a lane's bits
+            // past its own window, and its own atom, are cleared once per chunk below
+            const int kend = (min(64, rowmax - cb) + U - 1) & ~(U - 1);
+            const int sb = rb + cb;
+            uint32_t mlo = 0, mhi = 0, blo_m = 0, bhi_m = 0;
+            for (int k = 0; k < kend; k += U) {
+                float dx[U], dy[U], dz[U];
+                int tt[U];
 #pragma unroll
-                for (int j = 0; j < PIS_BUILD_UNROLL; ++j) {
-                    const int sl = rb + cb + k + j;
-                    ok[j] = k + j < kmax && sl < re && sl != self;
-                    const int sli = ok[j] ? sl : SCAP;
+                for (int j = 0; j < U; ++j) {
+                    const int sli = min(sb + k + j, SCAP);  // stays inside the staging
                     dx[j] = Px[sli] - qx32;
                     dy[j] = Py[sli] - qy32;
                     dz[j] = Pz[sli] - qz32;
                     tt[j] = MULTI ? (int)Pt[sli] : 0;
                 }
 #pragma unroll
-                for (int j = 0; j < PIS_BUILD_UNROLL; ++j) {
+                for (int j = 0; j < U; ++j) {
                     if (PAIRMI) {
                         dx[j] = mi_fast32(dx[j], L32[0], Li32[0]);
                         dy[j] = mi_fast32(dy[j], L32[1], Li32[1]);
@@ -644,47 +642,64 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
                         blo = tt[j] == 3 ? tlo[3] : blo;
                         bhi = tt[j] == 3 ? thi[3] : bhi;
                     }
-                    const bool in = ok[j] && r2 <= blo;
-                    band |= ok[j] && !in && r2 <= bhi;
-                    const uint32_t bit = in ? 1u : 0u;
-                    if (k + j < 32)
-                        mlo |= bit << (k + j);
-                    else
-                        mhi |= bit << (k + j - 32);
+                    const uint32_t in = r2 <= blo ? 1u : 0u;
+                    const uint32_t bd = (r2 > blo && r2 <= bhi) ? 1u : 0u;
+                    if (k + j < 32) {
+                        mlo |= in << (k + j);
+                        blo_m |= bd << (k + j);
+                    } else {
+                        mhi |= in << (k + j - 32);
+                        bhi_m |= bd << (k + j - 32);
+                    }
                 }
             }
-            if (__any(band)) {
+            {
+                // keep [sb, re) without the atom itself
+                const int nv = max(0, min(64, re - sb));
+                const uint32_t vlo = nv >= 32 ? 0xffffffffu : ((1u << nv) - 1u);
+                const uint32_t vhi = nv >= 64 ? 0xffffffffu : (nv <= 32 ? 0u : ((1u << (nv - 32)) - 1u));
+                uint32_t slo = 0xffffffffu, shi = 0xffffffffu;
+                const int ds = self - sb;
+                if (ds >= 0 && ds < 32) slo = ~(1u << ds);
+                if (ds >= 32 && ds < 64) shi = ~(1u << (ds - 32));
+                mlo &= vlo & slo;
+                mhi &= vhi & shi;
+                blo_m &= vlo & slo;
+                bhi_m &= vhi & shi;
+            }
+            if (__any((blo_m | bhi_m) != 0u)) {
                 // rare: settle the band candidates with the reference predicate
-                for (int k = 0; k < kmax && band; ++k) {
-                    const int sl = rb + cb + k;
-                    if (sl >= re || sl == self) continue;
-                    float dx = Px[sl] - qx32, dy = Py[sl] - qy32, dz = Pz[sl] - qz32;
-                    if (PAIRMI) {
-                        dx = mi_fast32(dx, L32[0], Li32[0]);
-                        dy = mi_fast32(dy, L32[1], Li32[1]);
-                        dz = mi_fast32(dz, L32[2], Li32[2]);
-                    }
-                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    const PairParam& p = ppl[MULTI ? ti * nt + Pt[sl] : 0];
-                    if (r2 > p.l_lo32 && r2 <= p.l_hi32 &&
-                        exact_within(s, st, slot_atom(pref, ecell, g.ne, sl), xi, yi, zi, p.r2_list)) {
-                        if (k < 32)
-                            mlo |= 1u << k;
-                        else
-                            mhi |= 1u << (k - 32);
+                for (int half = 0; half < 2; ++half) {
+                    uint32_t bm = half ? bhi_m : blo_m;
+                    while (bm) {
+                        const int kk = __builtin_ctz(bm) + 32 * half;
+                        bm &= bm - 1u;
+                        const int sl = sb + kk;
+                        const PairParam& p = ppl[MULTI ? ti * nt + Pt[sl] : 0];
+                        if (exact_within(s, st, slot_atom(pref, ecell, g.ne, sl), xi, yi, zi, p.r2_list)) {
+                            if (kk < 32)
+                                mlo |= 1u << kk;
+                            else
+                                mhi |= 1u << (kk - 32);
+                        }
                     }
                 }
             }
-            uint64_t mask = ((uint64_t)mhi << 32) | mlo;
-            while (mask) {
-                const int b = __builtin_ctzll(mask);
-                mask &= mask - 1;
-                lo = (lo >> 16) | (hi << 48);
-                hi = (hi >> 16) | ((uint64_t)((uint32_t)(rb + cb + b) << 3) << 48);
-                ++cnt;
-                if ((cnt & 7) == 0 && (cnt >> 3) <= maxg)
-                    out[(size_t)((cnt >> 3) - 1) * s.npad] =
-                        make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+            // accepted slots, ascending, into the 8-entry shift register (w0 oldest)
+            for (int half = 0; half < 2; ++half) {
+                uint32_t m = half ? mhi : mlo;
+                const uint32_t base = (uint32_t)(sb + 32 * half);
+                while (m) {
+                    const uint32_t b = (uint32_t)__builtin_ctz(m);
+                    m &= m - 1u;
+                    w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
+                    w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
+                    w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
+                    w3 = __builtin_amdgcn_alignbit((base + b) << 3, w3, 16);
+                    ++cnt;
+                    if ((cnt & 7) == 0 && (cnt >> 3) <= maxg)
+                        out[(size_t)((cnt >> 3) - 1) * s.npad] = make_uint4(w0, w1, w2, w3);
+                }
             }
         }
     }
@@ -692,13 +707,13 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
     const int n_real = cnt;
     if (cnt & 7) {
         while (cnt & 7) {
-            lo = (lo >> 16) | (hi << 48);
-            hi = (hi >> 16) | ((uint64_t)(SCAP << 3) << 48);
+            w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
+            w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
+            w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
+            w3 = __builtin_amdgcn_alignbit((uint32_t)(SCAP << 3), w3, 16);
             ++cnt;
         }
-        if ((cnt >> 3) <= maxg)
-            out[(size_t)((cnt >> 3) - 1) * s.npad] =
-                make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+        if ((cnt >> 3) <= maxg) out[(size_t)((cnt >> 3) - 1) * s.npad] = make_uint4(w0, w1, w2, w3);
     }
     s.nnb[a] = (uint16_t)min(n_real, s.maxnb);
     return n_real;
