@@ -57,16 +57,18 @@ def list_mean(spec):
 
 
 def force_kernel_bytes_per_atom(nbar):
-    """Algorithmic HBM bytes of one force launch per atom: the u16 list entries
-    (2 B each) and their count (2 B), position (24 B) and type (1 B) of the atom, old
-    force and velocity read (48 B), new force and velocity written (48 B).  Neighbour
+    """Algorithmic HBM bytes of one force launch per atom in a run's steady state: the
+    u16 list entries (2 B each) and their count (2 B), position (24 B) and type (1 B) of
+    the atom, old force and velocity read (48 B), the skin-check reference position read
+    (24 B), new force, velocity and the next step's position written (72 B).  Neighbour
     positions are staged once per tile from L2/MALL and counted at zero."""
-    return 2.0 * nbar + 2 + 24 + 1 + 48 + 48
+    return 2.0 * nbar + 2 + 24 + 1 + 48 + 24 + 72
 
 
 def step_bytes_per_atom(nbar):
-    """Whole step: force launch + position update (x, v, F, type, x_ref read; x written)."""
-    return force_kernel_bytes_per_atom(nbar) + (24 * 3 + 1 + 24 + 24)
+    """Whole steady-state step: the force launch, which also advances the positions (the
+    position-update kernel runs only on a call's first step)."""
+    return force_kernel_bytes_per_atom(nbar)
 
 
 def measured_traffic(natoms):

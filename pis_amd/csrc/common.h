@@ -35,7 +35,10 @@ enum {
     CTL_STEP = 7,      // thermo record index
     CTL_MAXSTAGE = 8,  // most atoms staged by one tile since the state was set
     CTL_LEFT = 9,      // NVE steps still to run: < 0 unlimited, 0 frozen (steps skip it)
-    CTL_COUNT = 10
+    CTL_REMAIN = 10,   // steps left in the current call (the force kernel advances the
+                       // positions of the next step while more steps follow)
+    CTL_REBUILD_NEXT = 11,  // skin check of those advanced positions
+    CTL_COUNT = 12
 };
 
 enum { MODE_TILE_REL = 0, MODE_RAW_MI = 1, MODE_WHOLE = 2 };

@@ -84,7 +84,9 @@ struct pis_ctx {
     bool multi = false;
     // captured step sequences, keyed by whether thermo is recorded
     bool use_graph = false;
-    hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};  // [record + 2 nvt]
+    // captured steps, [record + 2 nvt + 4 steady]: a call's first step (with the position
+    // update kernel) and its later NVE steps (positions already advanced by the force kernel)
+    hipGraphExec_t graph[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     bool nhc_ready = false;  // pis_nvt_init has set every system's chain
     // staging classes after the first run their list build and force on streams of their
     // own (default priority), forked from and joined back into `stream` by events
@@ -131,5 +133,10 @@ pis_status ctx_add(pis_ctx* c, Plan&& plan);
 pis_status ctx_remove(pis_ctx* c, std::vector<size_t> idx);
 // Device bytes a system with this plan occupies (its allocation plus `thermo_records`).
 size_t system_bytes(const Plan& p, int thermo_records);
+// Enqueues `nsteps` steps (record: thermo per step; nvt: the chain is applied) of every
+// system on the context's stream, from captured graphs when use_graph is set; the first
+// step runs the position-update kernel, later NVE steps find their positions advanced by
+// the previous step's force kernel.
+pis_status enqueue_steps(pis_ctx* c, uint64_t nsteps, bool record, bool nvt);
 
 }  // namespace pis

@@ -893,6 +893,61 @@ static pis_status batch_compute(pis_ctx* c, pis_thermo* out /* [nsys] or null */
     return PIS_OK;
 }
 
+// One step's launches; `first`: the position-update kernel runs (a call's first step,
+// and every NVT step, whose velocity scaling needs the chain update in between).
+static pis_status enqueue_one(pis_ctx* c, bool first, bool record, bool nvt, bool timed)
+{
+    const int ns = (int)c->sys.size();
+    if (nvt) launch_nhc_pre(c->stream, c->d_sys, ns);
+    if (first || nvt) launch_integrate_pos(c->stream, c->d_sys, c->d_atom_work, c->n_atom_work);
+    launch_rebuild_sort(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work);
+    if (timed) {
+        // timed: every list build first, then the force launches between events
+        PIS_TRY(enqueue_classes(c, true, -1));
+        PIS_TRY(launch_force_timed(c, 1));
+    } else {
+        PIS_TRY(enqueue_classes(c, true, 1));
+        c->force_launches++;
+    }
+    launch_reduce(c->stream, c->d_sys, ns, 1, record ? 1 : 0, 1);
+    return PIS_OK;
+}
+
+pis_status pis::enqueue_steps(pis_ctx* c, uint64_t nsteps, bool record, bool nvt)
+{
+    const int ns = (int)c->sys.size();
+    launch_set_ctl(c->stream, c->d_sys, ns, CTL_REMAIN, (int)std::min<uint64_t>(nsteps, (uint64_t)INT32_MAX));
+    if (c->use_graph && !c->timing) {
+        for (int steady = 0; steady < 2; ++steady) {
+            const int gi = (record ? 1 : 0) + (nvt ? 2 : 0) + 4 * steady;
+            if (c->graph[gi] || (steady && nsteps < 2)) continue;
+            hipGraph_t g = nullptr;
+            PIS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            const uint64_t launches_before = c->force_launches;
+            pis_status cs = enqueue_one(c, !steady, record, nvt, false);
+            c->force_launches = launches_before;
+            hipError_t ce = hipStreamEndCapture(c->stream, &g);
+            if (cs != PIS_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return cs;
+            }
+            if (ce != hipSuccess) return fail(PIS_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+            hipError_t e = hipGraphInstantiate(&c->graph[gi], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return fail(PIS_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+        }
+        const int g0 = (record ? 1 : 0) + (nvt ? 2 : 0);
+        for (uint64_t t = 0; t < nsteps; ++t) PIS_HIP(hipGraphLaunch(c->graph[g0 + (t ? 4 : 0)], c->stream));
+        c->force_launches += nsteps;
+    } else {
+        for (uint64_t t = 0; t < nsteps; ++t) {
+            PIS_TRY(enqueue_one(c, t == 0, record, nvt, c->timing));
+            if ((t & 255) == 0) PIS_HIP(hipGetLastError());
+        }
+    }
+    return PIS_OK;
+}
+
 // nsteps of verlet_step_nve (nvt = false) or verlet_step_nvt_nhc (nvt = true, chain set
 // by pis_nvt_init) for every system, state resident on the device.
 static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_thermo* out, bool nvt = false)
@@ -915,49 +970,7 @@ static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_
     launch_set_ctl(c->stream, c->d_sys, ns, CTL_STEP, 0);
     launch_set_ctl(c->stream, c->d_sys, ns, CTL_LEFT, -1);
     if (nvt) launch_nhc_run(c->stream, c->d_sys, ns, nsteps);
-    const int rec = out ? 1 : 0;
-    const int gi = rec + (nvt ? 2 : 0);
-    if (c->use_graph && !c->timing) {
-        // one captured step (every kernel's arguments are step-invariant: the thermo
-        // record index, the live buffer set and the chain state live on the device),
-        // replayed nsteps times
-        if (!c->graph[gi]) {
-            hipGraph_t g = nullptr;
-            PIS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            if (nvt) launch_nhc_pre(c->stream, c->d_sys, ns);
-            launch_integrate_pos(c->stream, c->d_sys, c->d_atom_work, c->n_atom_work);
-            launch_rebuild_sort(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work);
-            pis_status cs = enqueue_classes(c, true, 1);
-            launch_reduce(c->stream, c->d_sys, ns, 1, rec, 1);
-            hipError_t ce = hipStreamEndCapture(c->stream, &g);
-            if (cs != PIS_OK) {
-                if (g) (void)hipGraphDestroy(g);
-                return cs;
-            }
-            if (ce != hipSuccess) return fail(PIS_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
-            hipError_t e = hipGraphInstantiate(&c->graph[gi], g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) return fail(PIS_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
-        }
-        for (uint64_t t = 0; t < nsteps; ++t) PIS_HIP(hipGraphLaunch(c->graph[gi], c->stream));
-        c->force_launches += nsteps;
-    } else {
-        for (uint64_t t = 0; t < nsteps; ++t) {
-            if (nvt) launch_nhc_pre(c->stream, c->d_sys, ns);
-            launch_integrate_pos(c->stream, c->d_sys, c->d_atom_work, c->n_atom_work);
-            launch_rebuild_sort(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work);
-            if (c->timing) {
-                // timed: every list build first, then the force launches between events
-                PIS_TRY(enqueue_classes(c, true, -1));
-                PIS_TRY(launch_force_timed(c, 1));
-            } else {
-                PIS_TRY(enqueue_classes(c, true, 1));
-                c->force_launches++;
-            }
-            launch_reduce(c->stream, c->d_sys, ns, 1, rec, 1);
-            if ((t & 255) == 0) PIS_HIP(hipGetLastError());
-        }
-    }
+    PIS_TRY(enqueue_steps(c, nsteps, out != nullptr, nvt));
     PIS_HIP(hipGetLastError());
     PIS_HIP(hipStreamSynchronize(c->stream));
     c->steps += nsteps;

@@ -6,6 +6,8 @@
 //
 // Per NVE step (one launch each, all systems of a batch together):
 //   k_integrate_pos   x += v dt + (F/m)/2 dt^2, wrap, skin check      (potential.rs:15-22)
+//                     (first step of a call, and every NVT step; otherwise the previous
+//                     step's force kernel has already done it)
 //   k_bin             cell of every atom, integer histogram           (neighbour_list.rs:9-42)
 //   k_scan            exclusive scan of the histogram per system
 //   k_scatter         atoms into cell slots (atomic slot, then...)
@@ -281,7 +283,7 @@ __global__ void k_store_aos(const SysDev* __restrict__ sys, int k, double* __res
     const SysDev& s = sys[k];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= s.n) return;
-    const State st = s.st[s.ctl[CTL_CUR]];
+    const State st = s.st[__builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR])];
     const uint32_t o = st.orig[p];
     if (pos) {
         pos[3 * o] = st.x[p];
@@ -316,7 +318,7 @@ __global__ __launch_bounds__(kAtomThreads) void k_integrate_pos(const SysDev* __
     const SysDev& s = sys[w.sys];
     const int a = w.idx + threadIdx.x;
     if (a >= s.n || s.ctl[CTL_LEFT] == 0) return;
-    const State st = s.st[s.ctl[CTL_CUR]];
+    const State st = s.st[__builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR])];
     const double m = s.mass[st.type[a]];
     const double dt = s.dt, dt2 = s.dt2;
     double vx = st.vx[a], vy = st.vy[a], vz = st.vz[a];
@@ -367,7 +369,7 @@ __global__ __launch_bounds__(kAtomThreads) void k_bin(const SysDev* __restrict__
         if (!s.ctl[CTL_REBUILD]) continue;
         const int a = w.idx + threadIdx.x;
         if (a >= s.n) continue;
-        const State st = s.st[s.ctl[CTL_CUR]];
+        const State st = s.st[__builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR])];
         // rcut_cells (neighbour_list.rs:17-34): c = floor(s n) as usize (saturating), % n
         const double r[3] = {st.x[a], st.y[a], st.z[a]};
         int c[3];
@@ -421,7 +423,7 @@ __global__ __launch_bounds__(kAtomThreads) void k_scatter(const SysDev* __restri
         if (!s.ctl[CTL_REBUILD]) continue;
         const int a = w.idx + threadIdx.x;
         if (a >= s.n) continue;
-        const State st = s.st[s.ctl[CTL_CUR]];
+        const State st = s.st[__builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR])];
         const int sc = s.atom_cell[a];
         const int p = s.cell_start[sc] + atomicAdd(&s.cell_fill[sc], 1);
         s.scat_src[p] = a;
@@ -443,7 +445,7 @@ __global__ __launch_bounds__(kAtomThreads) void k_rank_permute(const SysDev* __r
         if (!s.ctl[CTL_REBUILD]) continue;
         const int p = w.idx + threadIdx.x;
         if (p >= s.n) continue;
-        const int cur = s.ctl[CTL_CUR];
+        const int cur = __builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR]);
         const State src = s.st[cur], dst = s.st[cur ^ 1];
         const int a = s.scat_src[p];
         const int sc = s.atom_cell[a];
@@ -736,7 +738,7 @@ __global__ __launch_bounds__(kTileThreads) void k_build_list(const SysDev* __res
         const WorkItem w = tiles[xcd_remap(b, ntw)];
         const SysDev& s = sys[w.sys];
         if (!s.ctl[CTL_REBUILD]) continue;
-        const State st = s.st[s.ctl[CTL_CUR] ^ 1];
+        const State st = s.st[__builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR]) ^ 1];
         const TileGeom g = tile_geom(s, w.idx);
         const int nt = s.ntypes;
         const bool rel = s.mode == MODE_TILE_REL;
@@ -930,7 +932,20 @@ __global__ __launch_bounds__(kTileThreads, MIXED ? PIS_FORCE_MIN_WAVES_MIXED : P
     const WorkItem w = tiles[xcd_remap(blockIdx.x, ntw)];
     const SysDev& s = sys[w.sys];
     if (integrate && s.ctl[CTL_LEFT] == 0) return;  // frozen system: the whole block leaves
-    const State st = s.st[s.ctl[CTL_CUR] ^ (s.ctl[CTL_REBUILD] ? 1 : 0)];
+    // read set: the live set, or the freshly sorted one on a rebuild step; a step writes
+    // its results into the other set (wt), which becomes live at the end of the step
+    // (control words are read with vector loads; readfirstlane makes the block-uniform
+    // choices scalar, so the state pointers stay in scalar registers)
+    const int cur = __builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR]);
+    const int rbf = __builtin_amdgcn_readfirstlane(s.ctl[CTL_REBUILD]);
+    const int rset = cur ^ (rbf ? 1 : 0);
+    const State st = s.st[rset];
+    const State wt = s.st[rset ^ 1];
+    const bool rebuilt = rbf != 0;
+    // while more steps of this call follow (and no thermostat must act in between), the
+    // next step's position update happens here, from the v and F just computed
+    const bool ahead = integrate && __builtin_amdgcn_readfirstlane(s.ctl[CTL_REMAIN]) > 1 &&
+                       __builtin_amdgcn_readfirstlane(s.ctl[CTL_LEFT]) != 1 && !s.nhc_on;
     const TileGeom g = tile_geom(s, w.idx);
     const bool rel = s.mode == MODE_TILE_REL;
     const double cx = (g.x0 + 0.5 * g.sx) * s.cellw[0];
@@ -945,7 +960,7 @@ __global__ __launch_bounds__(kTileThreads, MIXED ? PIS_FORCE_MIN_WAVES_MIXED : P
         total = stage_tile<SCAP, STAGE_RAW>(s, st, g, cx, cy, cz, Px, Py, Pz, Pt, pref, ecell, stmp);
     double sum[4] = {0.0, 0.0, 0.0, 0.0};
     if (total <= SCAP) {
-        const PairParam p0 = ppl[0];
+        const PairParam p0 = s.pp[0];  // uniform: scalar loads, no vector registers
         const int a0 = s.cell_start[g.c0], a1 = s.cell_start[g.c1];
         for (int a = a0 + (int)threadIdx.x; a < a1; a += blockDim.x) {
             const double xi = st.x[a], yi = st.y[a], zi = st.z[a];
@@ -970,6 +985,8 @@ __global__ __launch_bounds__(kTileThreads, MIXED ? PIS_FORCE_MIN_WAVES_MIXED : P
                     atom_forces<SCAP, MIXED, true, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
             }
             const double m = s.mass[ti];
+            // positions again from memory (L2): not held in registers across the pair loop
+            const double xa = st.x[a], ya = st.y[a], za = st.z[a];
             double vx = st.vx[a], vy = st.vy[a], vz = st.vz[a];
             if (integrate) {
                 // v += ((a_t + a_{t+dt}) 0.5) dt  (potential.rs:28-30)
@@ -983,16 +1000,54 @@ __global__ __launch_bounds__(kTileThreads, MIXED ? PIS_FORCE_MIN_WAVES_MIXED : P
                     vy = vy * sc;
                     vz = vz * sc;
                 }
-                st.vx[a] = vx;
-                st.vy[a] = vy;
-                st.vz[a] = vz;
+                wt.vx[a] = vx;
+                wt.vy[a] = vy;
+                wt.vz[a] = vz;
+                wt.fx[a] = acc.fx;
+                wt.fy[a] = acc.fy;
+                wt.fz[a] = acc.fz;
+                if (rebuilt) {  // the other set still has the order before the sort
+                    wt.type[a] = (uint8_t)ti;
+                    wt.orig[a] = st.orig[a];
+                }
+                if (ahead) {
+                    // the next step's x += v dt + (F/m)/2 dt^2, wrap and skin check, as
+                    // k_integrate_pos does them (potential.rs:15-22, simulation_box.rs:29-42)
+                    const double dt2 = s.dt2;
+                    double x = xa + (vx * dt + (acc.fx / m) * 0.5 * dt2);
+                    double y = ya + (vy * dt + (acc.fy / m) * 0.5 * dt2);
+                    double z = za + (vz * dt + (acc.fz / m) * 0.5 * dt2);
+                    double sx = x * s.Linv[0], sy = y * s.Linv[1], sz = z * s.Linv[2];
+                    sx = sx - floor(sx);
+                    sy = sy - floor(sy);
+                    sz = sz - floor(sz);
+                    x = s.L[0] * sx;
+                    y = s.L[1] * sy;
+                    z = s.L[2] * sz;
+                    wt.x[a] = x;
+                    wt.y[a] = y;
+                    wt.z[a] = z;
+                    const double ddx = mi_fast(x - s.rx[a], s.L[0], s.Linv[0]);
+                    const double ddy = mi_fast(y - s.ry[a], s.L[1], s.Linv[1]);
+                    const double ddz = mi_fast(z - s.rz[a], s.L[2], s.Linv[2]);
+                    const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                    if (!(d2 <= s.half_skin2)) {  // also catches NaN
+                        atomicOr(&s.ctl[CTL_REBUILD_NEXT], 1);
+                        if (d2 != d2) atomicOr(&s.ctl[CTL_NAN], 1);
+                    }
+                } else {
+                    wt.x[a] = xa;
+                    wt.y[a] = ya;
+                    wt.z[a] = za;
+                }
+            } else {
+                st.fx[a] = acc.fx;
+                st.fy[a] = acc.fy;
+                st.fz[a] = acc.fz;
             }
-            st.fx[a] = acc.fx;
-            st.fy[a] = acc.fy;
-            st.fz[a] = acc.fz;
             sum[0] += acc.pe;
             sum[1] += 0.5 * m * ((vx * vx + vy * vy) + vz * vz);
-            sum[2] += (xi * acc.fx + yi * acc.fy) + zi * acc.fz;
+            sum[2] += (xa * acc.fx + ya * acc.fy) + za * acc.fz;
             sum[3] += acc.wp;
         }
     } else if (threadIdx.x == 0) {
@@ -1112,12 +1167,21 @@ __global__ __launch_bounds__(1024) void k_reduce(const SysDev* __restrict__ sys,
         s.ctl[CTL_STEP] = k + 1;
     }
     if (!isfinite(pe) || !isfinite(ke)) s.ctl[CTL_NAN] = 1;
-    if (flip && s.ctl[CTL_REBUILD]) {
+    if (step) {
+        // the step wrote into the set it did not read: that set becomes live; the skin
+        // check of positions advanced ahead decides the next step's rebuild
+        const int rb = s.ctl[CTL_REBUILD] ? 1 : 0;
+        s.ctl[CTL_CUR] = s.ctl[CTL_CUR] ^ rb ^ 1;
+        s.ctl[CTL_NREBUILD] += rb;
+        s.ctl[CTL_REBUILD] = s.ctl[CTL_REBUILD_NEXT];
+        s.ctl[CTL_REBUILD_NEXT] = 0;
+        if (s.ctl[CTL_REMAIN] > 0) s.ctl[CTL_REMAIN] -= 1;
+        if (s.ctl[CTL_LEFT] > 0) s.ctl[CTL_LEFT] -= 1;
+    } else if (flip && s.ctl[CTL_REBUILD]) {
         s.ctl[CTL_CUR] ^= 1;
         s.ctl[CTL_REBUILD] = 0;
         s.ctl[CTL_NREBUILD] += 1;
     }
-    if (step && s.ctl[CTL_LEFT] > 0) s.ctl[CTL_LEFT] -= 1;
 }
 
 // Every system's control words into one array [nsys][CTL_COUNT] (one copy to the host).
@@ -1153,7 +1217,7 @@ __global__ __launch_bounds__(kTileThreads) void k_list_to_orig(const SysDev* __r
     __shared__ int stmp[16];
     const WorkItem w = tiles[blockIdx.x];
     const SysDev& s = sys[w.sys];
-    const State st = s.st[s.ctl[CTL_CUR]];
+    const State st = s.st[__builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR])];
     const TileGeom g = tile_geom(s, w.idx);
     const int total = stage_tile<SCAP, STAGE_INDEX, double>(s, st, g, 0.0, 0.0, 0.0, Px, nullptr, nullptr,
                                                     nullptr, pref, ecell, stmp);

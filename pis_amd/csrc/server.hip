@@ -142,24 +142,7 @@ pis_status run_round(pis_server* sv, int steps)
     pis_ctx* c = sv->ctx;
     const int ns = (int)c->sys.size();
     launch_set_ctl(c->stream, c->d_sys, ns, CTL_STEP, 0);
-    if (!c->graph[1]) {
-        hipGraph_t g = nullptr;
-        PIS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        launch_integrate_pos(c->stream, c->d_sys, c->d_atom_work, c->n_atom_work);
-        launch_rebuild_sort(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work);
-        pis_status cs = enqueue_classes(c, true, 1);
-        launch_reduce(c->stream, c->d_sys, ns, 1, 1, 1);
-        hipError_t ce = hipStreamEndCapture(c->stream, &g);
-        if (cs != PIS_OK) {
-            if (g) (void)hipGraphDestroy(g);
-            return cs;
-        }
-        if (ce != hipSuccess) return fail(PIS_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
-        hipError_t e = hipGraphInstantiate(&c->graph[1], g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) return fail(PIS_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
-    }
-    for (int t = 0; t < steps; ++t) PIS_HIP(hipGraphLaunch(c->graph[1], c->stream));
+    PIS_TRY(enqueue_steps(c, (uint64_t)steps, true, false));
     PIS_HIP(hipGetLastError());
     c->steps += (uint64_t)steps;
     return PIS_OK;
