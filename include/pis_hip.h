@@ -210,6 +210,12 @@ pis_status pis_nvt_init(pis_ctx* ctx, const pis_nhc_params* params);
 pis_status pis_verlet_step_nvt(pis_ctx* ctx, double dt, uint64_t nsteps, pis_thermo* per_step_out);
 pis_status pis_batch_verlet_step_nvt(pis_ctx* ctx, const double* dts, uint64_t nsteps,
                                      pis_thermo* thermo_out);
+/* Steps ramp_step0 .. ramp_step0 + nsteps - 1 of a run of ramp_total steps: the target
+ * ramp of Simulation::step (simulation.rs:53-57) continues across calls, so a run can be
+ * advanced in pieces (between trajectory dumps) with the reference's temperatures. */
+pis_status pis_batch_verlet_step_nvt_ramp(pis_ctx* ctx, const double* dts, uint64_t nsteps,
+                                          uint64_t ramp_step0, uint64_t ramp_total,
+                                          pis_thermo* thermo_out);
 /* The chain's thermostat velocities xi[chain] and current target temperature of system
  * `sys` (NHThermostatChain::xi / target_temperature, nvt.rs:9-13), for callers that keep a
  * host copy of the chain (either pointer may be NULL). */

@@ -176,6 +176,8 @@ SIGNATURES = {
     "pis_verlet_step_nvt": (C.c_int, [_P, C.c_double, C.c_uint64, C.POINTER(Thermo)]),
     "pis_batch_verlet_step_nvt": (C.c_int, [_P, _DP, C.c_uint64, C.POINTER(Thermo)]),
     "pis_nvt_get_chain": (C.c_int, [_P, C.c_uint32, _DP, _DP]),
+    "pis_batch_verlet_step_nvt_ramp": (
+        C.c_int, [_P, _DP, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(Thermo)]),
     "pis_server_create": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(_P)]),
     "pis_server_submit": (
         C.c_int,

@@ -950,7 +950,8 @@ pis_status pis::enqueue_steps(pis_ctx* c, uint64_t nsteps, bool record, bool nvt
 
 // nsteps of verlet_step_nve (nvt = false) or verlet_step_nvt_nhc (nvt = true, chain set
 // by pis_nvt_init) for every system, state resident on the device.
-static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_thermo* out, bool nvt = false)
+static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_thermo* out, bool nvt = false,
+                            uint64_t ramp_step0 = 0, uint64_t ramp_total = 0)
 {
     if (!c) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
     if (!dts) return fail(PIS_ERR_INVALID_ARGUMENT, "null timestep array");
@@ -969,7 +970,12 @@ static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_
     const int ns = (int)c->sys.size();
     launch_set_ctl(c->stream, c->d_sys, ns, CTL_STEP, 0);
     launch_set_ctl(c->stream, c->d_sys, ns, CTL_LEFT, -1);
-    if (nvt) launch_nhc_run(c->stream, c->d_sys, ns, nsteps);
+    if (nvt) {
+        if (ramp_total == 0) ramp_total = nsteps;
+        if (ramp_step0 + nsteps > ramp_total)
+            return fail(PIS_ERR_INVALID_ARGUMENT, "the steps run past the end of the temperature ramp");
+        launch_nhc_run(c->stream, c->d_sys, ns, ramp_step0, ramp_total);
+    }
     PIS_TRY(enqueue_steps(c, nsteps, out != nullptr, nvt));
     PIS_HIP(hipGetLastError());
     PIS_HIP(hipStreamSynchronize(c->stream));
@@ -1170,6 +1176,13 @@ pis_status pis_verlet_step_nvt(pis_ctx* ctx, double dt, uint64_t nsteps, pis_the
 pis_status pis_batch_verlet_step_nvt(pis_ctx* ctx, const double* dts, uint64_t nsteps, pis_thermo* thermo_out)
 {
     return batch_run(ctx, dts, nsteps, thermo_out, true);
+}
+
+pis_status pis_batch_verlet_step_nvt_ramp(pis_ctx* ctx, const double* dts, uint64_t nsteps, uint64_t ramp_step0,
+                                          uint64_t ramp_total, pis_thermo* thermo_out)
+{
+    if (ramp_total == 0) return fail(PIS_ERR_INVALID_ARGUMENT, "ramp_total must be > 0");
+    return batch_run(ctx, dts, nsteps, thermo_out, true, ramp_step0, ramp_total);
 }
 
 pis_status pis_get_neighbor_list(pis_ctx* ctx, uint64_t* offsets, uint32_t* idx, uint64_t cap, uint64_t* total)

@@ -26,7 +26,7 @@ void launch_reduce(hipStream_t st, const SysDev* d_sys, int nsys, int flip, int 
 void launch_gather_ctl(hipStream_t st, const SysDev* d_sys, int nsys, int32_t* out);
 // NVT: start-of-step chain update and velocity scale; reset of the run's step counter.
 void launch_nhc_pre(hipStream_t st, const SysDev* d_sys, int nsys);
-void launch_nhc_run(hipStream_t st, const SysDev* d_sys, int nsys, uint64_t total);
+void launch_nhc_run(hipStream_t st, const SysDev* d_sys, int nsys, uint64_t step0, uint64_t total);
 void launch_flip(hipStream_t st, const SysDev* d_sys, int nsys);
 void launch_list_to_orig(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work,
                          int n_tile_work, int scap, uint32_t* out, uint32_t* counts);

@@ -1121,11 +1121,11 @@ __global__ __launch_bounds__(1024) void k_nhc_pre(const SysDev* __restrict__ sys
 }
 
 // Step counter and length of an NVT run (the target ramp's i and total).
-__global__ void k_nhc_run(const SysDev* __restrict__ sys, int nsys, double total)
+__global__ void k_nhc_run(const SysDev* __restrict__ sys, int nsys, double step0, double total)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nsys || !sys[k].nhc_on) return;
-    sys[k].nhc[NHC_STEP] = 0.0;
+    sys[k].nhc[NHC_STEP] = step0;
     sys[k].nhc[NHC_TOTAL] = total;
 }
 
@@ -1354,9 +1354,10 @@ void launch_nhc_pre(hipStream_t st, const SysDev* d_sys, int nsys)
     hipLaunchKernelGGL(k_nhc_pre, dim3(nsys), dim3(1024), 0, st, d_sys);
 }
 
-void launch_nhc_run(hipStream_t st, const SysDev* d_sys, int nsys, uint64_t total)
+void launch_nhc_run(hipStream_t st, const SysDev* d_sys, int nsys, uint64_t step0, uint64_t total)
 {
-    hipLaunchKernelGGL(k_nhc_run, dim3((nsys + 255) / 256), dim3(256), 0, st, d_sys, nsys, (double)total);
+    hipLaunchKernelGGL(k_nhc_run, dim3((nsys + 255) / 256), dim3(256), 0, st, d_sys, nsys, (double)step0,
+                       (double)total);
 }
 
 void launch_gather_ctl(hipStream_t st, const SysDev* d_sys, int nsys, int32_t* out)

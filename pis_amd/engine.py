@@ -153,12 +153,17 @@ class Engine:
         prm = _capi.NHCParams(float(t_start), float(t_end), float(tau), int(chain), 0)
         check(lib().pis_nvt_init(self._ptr, C.byref(prm)))
 
-    def run_nvt(self, dt, nsteps: int, record: bool = True):
+    def run_nvt(self, dt, nsteps: int, record: bool = True, ramp=None):
         """nsteps of verlet_step_nvt_nhc for every system; thermo [nsteps, nsys, 7] with
-        etot = PE + KE + chain KE."""
+        etot = PE + KE + chain KE.  ramp=(first step, total steps) continues the target
+        temperature ramp of a longer run (default: this call is the whole run)."""
         dts = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (self.nsys,)))
         recs = (_capi.Thermo * (nsteps * self.nsys))() if record else None
-        check(lib().pis_batch_verlet_step_nvt(self._ptr, _dptr(dts), nsteps, recs))
+        if ramp is None:
+            check(lib().pis_batch_verlet_step_nvt(self._ptr, _dptr(dts), nsteps, recs))
+        else:
+            check(lib().pis_batch_verlet_step_nvt_ramp(self._ptr, _dptr(dts), nsteps, int(ramp[0]), int(ramp[1]),
+                                                       recs))
         if not record:
             return None
         return thermo_to_array(recs).reshape(nsteps, self.nsys, 7)

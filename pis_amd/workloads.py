@@ -233,60 +233,16 @@ def ka_ensemble(n_systems: int = 512, seed: int = 7, nmin: int = 256, nmax: int 
 
 def read_lammps_data(path: str) -> SystemSpec:
     """LAMMPS-style data file as the reference reads it (run_read_data,
-    src/readers/input_file/commands.rs:147-384): header counts, box bounds, Masses,
-    PairCoeffs (a line `i eps sigma [rcut]`, rcut defaulting to 2.5 sigma), Atoms and an
-    optional Velocities section.  Zero tilts, periodic everywhere."""
-    n_atoms = 0
-    ntypes = 0
-    bounds = {"x": (0.0, 1.0), "y": (0.0, 1.0), "z": (0.0, 1.0)}
-    masses = []
-    pairs = []
-    pos = vel = types = None
-    section = ""
-    with open(path) as fh:
-        for raw in fh:
-            line = raw.split("#", 1)[0].strip()
-            if not line:
-                continue
-            w = line.split()
-            if w[0] in ("Masses", "Atoms", "PairCoeffs", "Velocities"):
-                section = w[0]
-                continue
-            if len(w) > 1 and w[1] == "atoms":
-                n_atoms = int(w[0])
-                pos = np.zeros((n_atoms, 3))
-                vel = None
-                types = np.zeros(n_atoms, dtype=np.uint32)
-                continue
-            if len(w) > 1 and w[1] == "atom":
-                ntypes = int(w[0])
-                masses = [0.0] * ntypes
-                continue
-            if len(w) > 2 and w[2] in ("xlo", "ylo", "zlo"):
-                bounds[w[2][0]] = (float(w[0]), float(w[1]))
-                continue
-            if section == "Masses":
-                masses[int(w[0]) - 1] = float(w[1])
-            elif section == "PairCoeffs":
-                i = int(w[0])
-                try:
-                    eps = float(w[1])
-                    sig = float(w[2])
-                    rc = float(w[3]) if len(w) > 3 else 2.5 * sig
-                    pairs.append((i, i, eps, sig, rc))
-                except ValueError:
-                    j = int(w[1])
-                    sig = float(w[3])
-                    pairs.append((i, j, float(w[2]), sig, float(w[4]) if len(w) > 4 else 2.5 * sig))
-            elif section == "Atoms":
-                k = int(w[0]) - 1
-                types[k] = int(w[1])
-                pos[k] = [float(w[2]), float(w[3]), float(w[4])]
-            elif section == "Velocities":
-                if vel is None:
-                    vel = np.zeros((n_atoms, 3))
-                k = int(w[0]) - 1
-                vel[k] = [float(w[1]), float(w[2]), float(w[3])]
-    box = SimulationBox.from_lammps_data(*bounds["x"], *bounds["y"], *bounds["z"])
-    return SystemSpec(positions=pos, type_ids=types, masses=np.array(masses), pairs=pairs, box=box,
-                      velocities=vel, kb=KB_REFERENCE, name=path)
+    src/readers/input_file/commands.rs:147-384; restated in pis_amd.script.read_data):
+    header counts, box bounds, Masses, PairCoeffs (a line `i eps sigma [rcut]`, rcut
+    defaulting to 2.5 sigma), Atoms and an optional Velocities section.  Zero tilts,
+    periodic everywhere.  Velocities are None when the file has no Velocities section."""
+    from . import script
+
+    ctx = script.Context()
+    script.read_data(path, ctx)
+    a = ctx.atoms
+    pairs = [(i, j, e, sg, r) for (i, j), (e, sg, r) in (ctx.table or {}).items() if i <= j]
+    vel = None if ctx.starting_velocity.start_velocity else a.velocities
+    return SystemSpec(positions=a.positions, type_ids=a.type_ids.astype(np.uint32), masses=a.masses, pairs=pairs,
+                      box=a.sim_box, velocities=vel, kb=KB_REFERENCE, name=path)

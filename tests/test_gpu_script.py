@@ -1,0 +1,61 @@
+"""GPU: an input script run end to end (System::read -> contextualize -> Simulation::run)
+with the device engine, against the oracle running the same system step by step: thermo
+lines and trajectory blocks."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from pis_amd import script as S
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.mark.parametrize("ensemble", ["nve", "nvt"])
+def test_script_run_matches_oracle(tmp_path, ensemble):
+    fix = "fix t all nvt temp 100.0 120.0 1.0\n" if ensemble == "nvt" else ""
+    inp = tmp_path / "in.pis"
+    inp.write_text(f"timestep 0.01\nread_data {os.path.join(GOLDEN, 'example.txt')}\n"
+                   f"velocity all create 100.0 7\n{fix}dump d all atom 10 traj.lammpstrj\nrun 40\n")
+    ctx = S.contextualize(S.read_script(str(inp), base_dir=str(tmp_path)))
+    spec = S.to_spec(ctx)
+    lines = []
+    th = S.run(ctx, "fp64", print_fn=lines.append)
+    assert len(lines) == 41 and lines[0].startswith("0 ")
+    # the oracle, step by step, with the trajectory at every 10th step
+    o = po.OSystem.from_spec(spec)
+    if ensemble == "nvt":
+        ref = o.run_nvt(0.01, 40, 100.0, 120.0, 1.0, 3, po.MGR_OFFSET, kb=spec.kb)
+        frames = None
+    else:
+        o.compute(po.MGR_OFFSET)
+        ref, frames = [], {}
+        for t in range(1, 41):
+            pe = o.verlet_step_nve(0.01, po.MGR_OFFSET)
+            ke = o.kinetic_energy()
+            ref.append([pe, ke, pe + ke, o.temperature(ke, spec.kb), o.pressure(ke)])
+            if t % 10 == 0:
+                frames[t] = o.pos.copy()
+        ref = np.array(ref)
+    scale = np.abs(ref[:, 0]).max()
+    assert np.abs(th[:, :3] - ref[:, :3]).max() <= 1e-9 * scale
+    for t in range(40):
+        words = lines[t + 1].split()
+        assert int(words[0]) == t + 1
+        assert np.allclose([float(w) for w in words[1:]], ref[t], rtol=0, atol=1.5e-3)
+    # trajectory: blocks at 0, 10, 20, 30, 40 in the reference's layout
+    text = (tmp_path / "traj.lammpstrj").read_text().splitlines()
+    n = spec.n
+    block = 9 + n
+    assert len(text) == 5 * block
+    for b, step in enumerate((0, 10, 20, 30, 40)):
+        head = text[b * block: b * block + 9]
+        assert head[0] == "ITEM: TIMESTEP" and head[1] == str(step) and head[3] == str(n)
+        rows = np.array([[float(v) for v in r.split()[2:]] for r in text[b * block + 9:(b + 1) * block]])
+        if frames is not None and step in frames:
+            L = spec.box.lengths
+            d = (rows - frames[step] + L / 2) % L - L / 2
+            assert np.abs(d).max() <= 1e-9

@@ -1,0 +1,268 @@
+"""The input script / data file reader and the trajectory writer (no GPU).
+
+The command cases restate the reference's own tests, src/tests/command_tests.rs (the only
+tests the reference has), against pis_amd.script: same inputs, same accepted values and
+the same PisError variants.  The number format of the trajectory file follows Rust's `{}`
+for f64, pinned here by its defining cases.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from pis_amd import script as S
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def ok(cmd, raw, ctx):
+    S.run_command(cmd, raw.split(), 0, ctx)
+
+
+def err(cmd, raw, ctx):
+    with pytest.raises(S.PisScriptError) as e:
+        S.run_command(cmd, raw.split(), 0, ctx)
+    return e.value
+
+
+@pytest.fixture
+def ctx():
+    return S.Context()
+
+
+# command_tests.rs:50-70
+@pytest.mark.parametrize("name", ["timestep", "run", "velocity", "read_data", "fix", "pair_style", "pair_coeff", "dump"])
+def test_known_commands(name):
+    assert S.command_from_str(name) is not None
+
+
+@pytest.mark.parametrize("name", ["unknown_command", "", "TIMESTEP", "TimeStep"])
+def test_unknown_commands(name):
+    assert S.command_from_str(name) is None
+
+
+# command_tests.rs:74-100
+def test_timestep(ctx):
+    ok("timestep", "0.001", ctx)
+    assert ctx.timestep == 0.001
+    e = err("timestep", "", ctx)
+    assert e.kind == "MissingArgument" and e.fields == {"line": 0}
+
+
+@pytest.mark.parametrize("bad", ["notanumber", "one", "0.1.2"])
+def test_timestep_float_parse_error(ctx, bad):
+    e = err("timestep", bad, ctx)
+    assert e.kind == "FloatParseError" and e.fields["string"] == bad
+
+
+# command_tests.rs:104-137
+def test_run(ctx):
+    ok("run", "5000", ctx)
+    assert ctx.steps == 5000
+    assert err("run", "", ctx).kind == "MissingArgument"
+    e = err("run", "-1", ctx)
+    assert e.kind == "NegativeValue" and e.fields == {"value": -1, "line": 0}
+
+
+@pytest.mark.parametrize("bad", ["1.5", "ten", "1e3"])
+def test_run_int_parse_error(ctx, bad):
+    e = err("run", bad, ctx)
+    assert e.kind == "IntParseError" and e.fields["string"] == bad
+
+
+# command_tests.rs:141-216
+def test_velocity_create(ctx):
+    ok("velocity", "all create 300.0 12345", ctx)
+    sv = ctx.starting_velocity
+    assert sv.group == "all" and sv.start_temperature == 300.0 and sv.seed == 12345
+    ok("velocity", "all create 300.0", ctx)
+    assert ctx.starting_velocity.seed == 0
+
+
+@pytest.mark.parametrize("raw,dist", [("all create 300.0 42 dist gaussian", "gaussian"),
+                                      ("all create 300.0 42 dist uniform", "uniform"),
+                                      ("all create 300.0 dist gaussian", "gaussian")])
+def test_velocity_dist(ctx, raw, dist):
+    ok("velocity", raw, ctx)
+    assert ctx.starting_velocity.dist == dist
+
+
+@pytest.mark.parametrize("raw,bad", [("all set 300.0 42", "set"), ("all scale 300.0 42", "scale"),
+                                     ("all create 300.0 42 badkeyword value", "badkeyword"),
+                                     ("all create 300.0 42 dist lorentzian", "lorentzian")])
+def test_velocity_invalid_argument(ctx, raw, bad):
+    e = err("velocity", raw, ctx)
+    assert e.kind == "InvalidArgument" and e.fields == {"string": bad, "line": 0}
+
+
+def test_velocity_missing_group(ctx):
+    assert err("velocity", "", ctx).kind == "MissingArgument"
+
+
+# command_tests.rs:220-253
+def test_pair_style_and_coeff(ctx):
+    e = err("pair_coeff", "1 1 0.238 3.405 8.5", ctx)
+    assert e.kind == "PotentialNotInitialized"
+    S.run_command("pair_style", ["lj/cut", "10.0"], 7, ctx)
+    assert ctx.potential_args.pair_style_args == ["lj/cut", "10.0"]
+    assert ctx.potential_args.pair_style_line == 7
+    ok("pair_coeff", "1 1 0.238 3.405 8.5", ctx)
+    ok("pair_coeff", "1 2 0.150 3.100 7.5", ctx)
+    assert ctx.potential_args.pair_coeff_args == [["1", "1", "0.238", "3.405", "8.5"],
+                                                  ["1", "2", "0.150", "3.100", "7.5"]]
+
+
+# command_tests.rs:257-292
+def test_dump(ctx):
+    ok("dump", "my_dump all atom 100 output.lammpstrj", ctx)
+    d = ctx.dump_args
+    assert (d.name, d.group, d.style, d.dump_step, d.file_name) == ("my_dump", "all", "atom", 100,
+                                                                   "output.lammpstrj")
+
+
+@pytest.mark.parametrize("raw", ["my_dump all atom 100", "my_dump all atom", "my_dump all"])
+def test_dump_missing(ctx, raw):
+    assert err("dump", raw, ctx).kind == "MissingArgument"
+
+
+@pytest.mark.parametrize("raw,bad", [("my_dump all atom every output.lammpstrj", "every"),
+                                     ("my_dump all atom 1.5 output.lammpstrj", "1.5")])
+def test_dump_int_parse(ctx, raw, bad):
+    e = err("dump", raw, ctx)
+    assert e.kind == "IntParseError" and e.fields["string"] == bad
+
+
+# command_tests.rs:296-356
+def test_fix_nvt_and_npt(ctx):
+    ok("fix", "myfix all nvt temp 300.0 300.0 0.1", ctx)
+    nh = ctx.nh_chain_args
+    assert nh["name"] == "myfix" and nh["group"] == "all" and nh["start_temperature"] == 300.0 and nh["tau"] == 0.1
+    assert ctx.ensemble == "nvt"
+    ok("fix", "myfix all npt temp 300.0 300.0 0.1 iso 2.0 2.0 0.5", ctx)
+    assert ctx.mtk_barostat_args["start_pressure"] == 2.0 and ctx.mtk_barostat_args["tau"] == 0.5
+    assert ctx.ensemble == "npt"
+
+
+@pytest.mark.parametrize("raw", ["myfix all", "myfix"])
+def test_fix_missing(ctx, raw):
+    assert err("fix", raw, ctx).kind == "MissingArgument"
+
+
+@pytest.mark.parametrize("raw,bad", [("myfix all nvt badkeyword 300.0 300.0 0.1", "badkeyword"),
+                                     ("myfix all npt unknown 1.0 1.0 1.0", "unknown")])
+def test_fix_invalid(ctx, raw, bad):
+    e = err("fix", raw, ctx)
+    assert e.kind == "InvalidArgument" and e.fields == {"string": bad, "line": 0}
+
+
+# command_tests.rs:360-502
+def _write(tmp_path, text):
+    p = tmp_path / "d.data"
+    p.write_text(text)
+    return str(p)
+
+
+HEAD = "{n} atoms\n1 atom types\n\n0.0 {L} xlo xhi\n0.0 {L} ylo yhi\n0.0 {L} zlo zhi\n\nMasses\n\n1 1.0\n\n"
+
+
+def test_read_data_populates_atoms_and_box(ctx, tmp_path):
+    p = _write(tmp_path, "# Simple test data file\n\n" + HEAD.format(n=3, L=10.0) +
+               "Atoms\n\n1 1 1.0 2.0 3.0\n2 1 4.0 5.0 6.0\n3 1 7.0 8.0 9.0\n")
+    S.read_data(p, ctx)
+    assert ctx.atoms.n_atoms == 3
+    assert ctx.atoms.positions[0, 0] == 1.0 and ctx.atoms.positions[1, 1] == 5.0
+
+
+def test_read_data_missing_file(ctx):
+    e = pytest.raises(S.PisScriptError, S.read_data, "/nonexistent/path/to/file.data", ctx).value
+    assert e.kind == "InputFileError" and e.fields["path"] == "/nonexistent/path/to/file.data"
+
+
+@pytest.mark.parametrize("bad_id", [0, 5])
+def test_read_data_invalid_atom_id(ctx, tmp_path, bad_id):
+    p = _write(tmp_path, HEAD.format(n=2, L=5.0) + f"Atoms\n\n1 1 0.0 0.0 0.0\n{bad_id} 1 1.0 1.0 1.0\n")
+    e = pytest.raises(S.PisScriptError, S.read_data, p, ctx).value
+    assert e.kind == "AtomCountMismatch" and e.fields == {"expected": 2, "found": bad_id}
+
+
+def test_read_data_velocities_turn_off_generation(ctx, tmp_path):
+    p = _write(tmp_path, HEAD.format(n=2, L=5.0) + "Atoms\n\n1 1 0.0 0.0 0.0\n2 1 1.0 1.0 1.0\n\n"
+               "Velocities\n\n1 0.1 0.2 0.3\n2 0.4 0.5 0.6\n")
+    S.read_data(p, ctx)
+    assert ctx.starting_velocity.start_velocity is False
+    np.testing.assert_array_equal(ctx.atoms.velocities, [[0.1, 0.2, 0.3], [0.4, 0.5, 0.6]])
+
+
+def test_read_data_pair_coeffs_populate_the_table(ctx, tmp_path):
+    p = _write(tmp_path, HEAD.format(n=2, L=5.0) + "PairCoeffs\n\n1 0.238 3.405 8.5\n\n"
+               "Atoms\n\n1 1 0.0 0.0 0.0\n2 1 1.0 1.0 1.0\n")
+    S.read_data(p, ctx)
+    assert ctx.table == {(1, 1): (0.238, 3.405, 8.5)}
+
+
+# ---------------------------------------------------------------- whole script
+
+
+def test_reference_example_script(tmp_path):
+    # the script says "read_data example/argon4000.txt", relative to where it is run
+    os.symlink(GOLDEN, tmp_path / "example")
+    ctx = S.read_script(os.path.join(GOLDEN, "input.pis"), base_dir=str(tmp_path))
+    assert ctx.timestep == 0.25 and ctx.steps == 3000
+    assert ctx.atoms.n_atoms == 4000
+    assert ctx.starting_velocity.start_temperature == 5.0 and ctx.starting_velocity.seed == 12345
+    assert ctx.ensemble == "npt"
+    assert ctx.nh_chain_args["end_temperature"] == 50.0
+    d = ctx.dump_args
+    assert (d.name, d.style, d.dump_step, d.file_name) == ("dp1", "atom", 10, "dump.lammpstrj")
+    assert ctx.table == {(1, 1): (0.238, 3.405, 8.5)}
+
+
+def test_unknown_command_and_comment_handling(tmp_path):
+    p = tmp_path / "in.pis"
+    p.write_text("# comment\ntimestep 0.5 # trailing\nrun\nbogus 1\n")
+    e = pytest.raises(S.PisScriptError, S.read_script, str(p)).value
+    assert e.kind == "UnknownCommand" and e.fields == {"command": "bogus", "line": 4}
+
+
+def test_contextualize_builds_lj_table_with_global_cutoff_and_reversed_keys_ignored(tmp_path):
+    data = tmp_path / "d.data"
+    data.write_text(HEAD.format(n=2, L=10.0).replace("1 atom types", "2 atom types").replace("1 1.0\n", "1 1.0\n2 2.0\n")
+                    + "Atoms\n\n1 1 1.0 1.0 1.0\n2 2 3.0 3.0 3.0\n")
+    p = tmp_path / "in.pis"
+    p.write_text(f"read_data {data}\npair_style lj/cut 2.5\npair_coeff 1 1 1.0 1.0\n"
+                 "pair_coeff 2 1 0.5 0.8 2.0\npair_coeff 1 2 0.7 0.9\nvelocity all create 1.0 3\n")
+    ctx = S.contextualize(S.read_script(str(p)))
+    assert ctx.table[(1, 1)] == (1.0, 1.0, 2.5)
+    assert ctx.table[(1, 2)] == (0.7, 0.9, 2.5)
+    spec = S.to_spec(ctx)
+    assert sorted(x[:2] for x in spec.pairs) == [(1, 1), (1, 2)]  # (2, 1) is never looked up
+    assert np.all(np.isfinite(spec.velocities)) and np.abs(spec.velocities).sum() > 0
+
+
+# ---------------------------------------------------------------- DumpTraj
+
+
+@pytest.mark.parametrize("x,text", [(0.0, "0"), (-0.0, "-0"), (52.6, "52.6"), (1.0, "1"), (1e-5, "0.00001"),
+                                    (1.5e20, "150000000000000000000"), (0.1 + 0.2, "0.30000000000000004"),
+                                    (5.26, "5.26"), (float("inf"), "inf"), (float("nan"), "NaN")])
+def test_rust_float_display(x, text):
+    assert S.rust_float(x) == text
+
+
+def test_dump_block_format(tmp_path):
+    from pis_amd.box import SimulationBox
+
+    box = SimulationBox.cubic(27.05)
+    p = tmp_path / "t.lammpstrj"
+    d = S.DumpTraj(str(p))
+    d.write_step(np.array([[0.0, 2.705, 1e-5], [13.525, 0.5, 27.0]]), np.array([1, 2]), box, 10)
+    d.close()
+    assert p.read_text().splitlines() == [
+        "ITEM: TIMESTEP", "10", "ITEM: NUMBER OF ATOMS", "2", "ITEM: BOX BOUNDS pp pp pp",
+        "0 27.05", "0 27.05", "0 27.05", "ITEM: ATOMS id type x y z",
+        "1 1 0 2.705 0.00001", "2 2 13.525 0.5 27"]
+
+
+def test_thermo_line_format():
+    # 2.0005 is stored as 2.000500000000000167, so it rounds up, as Rust's {:.3} does
+    assert S.thermo_line(12, [-1.23456, 2.0005, 0.7774999, 3.14159, -0.0004]) == "12 -1.235 2.001 0.777 3.142 -0.000"
