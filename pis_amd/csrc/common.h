@@ -94,8 +94,7 @@ struct SysDev {
     State st[2];
     int32_t* ctl;
     double *rx, *ry, *rz;  // positions at the last build (sorted order)
-    int32_t* cell_count;   // [ncell]
-    int32_t* cell_fill;    // [ncell]
+    int32_t* cell_count;   // [ncell] atoms per cell while sorting; zero between rebuilds
     int32_t* cell_start;   // [ncell+1], sorted (tile-major) cell ids
     int32_t* nat2sorted;   // [ncell]: natural (cz*ny+cy)*nx+cx -> sorted id
     int32_t* tile_cell0;   // [ntile+1]

@@ -309,7 +309,7 @@ namespace {
 // Carving of one system's allocation (offsets of every array, 256-byte aligned).
 struct Layout {
     size_t st[2][11];
-    size_t r, ctl, cc, cf, cs, n2s, tc, ac, ss, sk, sx, nbr, nnb, part, pp, m, nhc;
+    size_t r, ctl, cc, cs, n2s, tc, ac, ss, sk, sx, nbr, nnb, part, pp, m, nhc;
     size_t total;
 };
 
@@ -331,7 +331,6 @@ Layout layout_of(const Plan& p)
     L.r = take(np * 8 * 3);
     L.ctl = take(64 * 4);
     L.cc = take(nc * 4);
-    L.cf = take(nc * 4);
     L.cs = take((nc + 1) * 4);
     L.n2s = take(nc * 4);
     L.tc = take(((size_t)p.ntile + 1) * 4);
@@ -363,7 +362,7 @@ pis_status pis::alloc_system(SysHost& h)
     const Layout Lo = layout_of(p);
     const size_t off = Lo.total;
     const auto& o_st = Lo.st;
-    const size_t o_r = Lo.r, o_ctl = Lo.ctl, o_cc = Lo.cc, o_cf = Lo.cf, o_cs = Lo.cs, o_n2s = Lo.n2s,
+    const size_t o_r = Lo.r, o_ctl = Lo.ctl, o_cc = Lo.cc, o_cs = Lo.cs, o_n2s = Lo.n2s,
                  o_tc = Lo.tc, o_ac = Lo.ac, o_ss = Lo.ss, o_sk = Lo.sk, o_nbr = Lo.nbr, o_nnb = Lo.nnb,
                  o_part = Lo.part, o_pp = Lo.pp, o_m = Lo.m;
     void* mem = nullptr;
@@ -384,7 +383,6 @@ pis_status pis::alloc_system(SysHost& h)
     d.rz = d.ry + np;
     d.ctl = (int32_t*)(b + o_ctl);
     d.cell_count = (int32_t*)(b + o_cc);
-    d.cell_fill = (int32_t*)(b + o_cf);
     d.cell_start = (int32_t*)(b + o_cs);
     d.nat2sorted = (int32_t*)(b + o_n2s);
     d.tile_cell0 = (int32_t*)(b + o_tc);

@@ -16,7 +16,9 @@
 //   k_build_list      full neighbour list at rcut+skin, reference predicate bit-for-bit
 //                     (lennard_jones.rs:344-416)
 //   k_force           LDS-staged pair force + fused second half of the Verlet update
-//                     (lennard_jones.rs:32-55, 419-455; potential.rs:26-30)
+//                     (lennard_jones.rs:32-55, 419-455; potential.rs:26-30), then the first
+//                     half of the next step when the call runs one (positions into the
+//                     other buffer set, wrap, skin check)
 //   k_reduce          fixed-order per-system PE/KE/virial sums      (properties.rs:17-65)
 // The five rebuild kernels return at once unless the system's rebuild flag is set.
 #include <math.h>
@@ -25,7 +27,8 @@
 
 // Tuning knobs fixed at build time: candidates of the force kernel whose LDS loads are
 // issued together (4 or 8); the minimum waves per SIMD its register allocation must allow
-// (launch-bounds occupancy target) per precision; the persistent grid of the list build.
+// (launch-bounds occupancy target) per precision; the grids (workgroups that loop over the
+// work items) of the per-atom sort kernels and of the list build.
 #ifndef PIS_FORCE_BATCH
 #define PIS_FORCE_BATCH 8
 #endif
@@ -37,6 +40,9 @@
 #endif
 #ifndef PIS_BUILD_UNROLL
 #define PIS_BUILD_UNROLL 4
+#endif
+#ifndef PIS_SORT_GRID_CAP
+#define PIS_SORT_GRID_CAP 1024
 #endif
 #ifndef PIS_BUILD_GRID_CAP
 #define PIS_BUILD_GRID_CAP (1 << 30)
@@ -102,6 +108,38 @@ __device__ int block_excl_scan(int v, int* tmp, int* total)
     __syncthreads();
     *total = tot;
     return add + x - v;
+}
+
+// Runs of equal, adjacent keys among the active lanes of a wave (the active lanes form a
+// prefix of the wave): this lane's run head (a lane id) and the run's length.  Atoms arrive
+// in the previous cell order, so the lanes of a wave span a few cells and one atomic per
+// run replaces one per atom.
+__device__ __forceinline__ void wave_runs(int key, bool active, int* head, int* len)
+{
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(key, 1, 64);
+    const uint64_t heads = __ballot(active && (lane == 0 || prev != key));
+    const uint64_t act = __ballot(active);
+    const uint64_t upto = (2ull << lane) - 1;  // lanes 0..lane (all ones for lane 63)
+    const int h = 63 - __clzll(heads & upto);
+    const uint64_t after = heads & ~upto;
+    const int end = after ? __ffsll((unsigned long long)after) - 1 : __popcll(act);
+    *head = h;
+    *len = end - h;
+}
+
+// Whether any work item this block will visit (items blockIdx.x + k gridDim.x, through
+// xcd_remap when REMAP) belongs to a system with its rebuild flag set.  One parallel probe:
+// on steps without a rebuild a block returns after two dependent loads instead of walking
+// its items one by one.
+template <bool REMAP>
+__device__ __forceinline__ bool block_rebuild_pending(const SysDev* __restrict__ sys,
+                                                      const WorkItem* __restrict__ work, int nwork)
+{
+    int flag = 0;
+    for (int b = blockIdx.x + (int)threadIdx.x * (int)gridDim.x; b < nwork; b += (int)(blockDim.x * gridDim.x))
+        flag |= sys[work[REMAP ? xcd_remap(b, nwork) : b].sys].ctl[CTL_REBUILD];
+    return __syncthreads_or(flag) != 0;
 }
 
 // Deterministic sum of 4 doubles over the block (<= 16 waves); result valid in thread 0.
@@ -363,116 +401,181 @@ __global__ __launch_bounds__(kAtomThreads) void k_integrate_pos(const SysDev* __
 __global__ __launch_bounds__(kAtomThreads) void k_bin(const SysDev* __restrict__ sys,
                                                        const WorkItem* __restrict__ work, int nwork)
 {
+    if (!block_rebuild_pending<false>(sys, work, nwork)) return;
     for (int b = blockIdx.x; b < nwork; b += gridDim.x) {
         const WorkItem w = work[b];
         const SysDev& s = sys[w.sys];
-        if (!s.ctl[CTL_REBUILD]) continue;
+        if (!s.ctl[CTL_REBUILD]) continue;  // uniform over the block: one system per item
         const int a = w.idx + threadIdx.x;
-        if (a >= s.n) continue;
-        const State st = s.st[__builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR])];
-        // rcut_cells (neighbour_list.rs:17-34): c = floor(s n) as usize (saturating), % n
-        const double r[3] = {st.x[a], st.y[a], st.z[a]};
-        int c[3];
+        const bool act = a < s.n;
+        int sc = 0;
+        if (act) {
+            const State st = s.st[__builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR])];
+            // rcut_cells (neighbour_list.rs:17-34): c = floor(s n) as usize (saturating), % n
+            const double r[3] = {st.x[a], st.y[a], st.z[a]};
+            int c[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const double f = floor((r[d] * s.Linv[d]) * (double)s.nc[d]);
-            const int v = f > 0.0 ? (f < 2.0e9 ? (int)f : 2000000000) : 0;
-            c[d] = v % s.nc[d];
+            for (int d = 0; d < 3; ++d) {
+                const double f = floor((r[d] * s.Linv[d]) * (double)s.nc[d]);
+                const int v = f > 0.0 ? (f < 2.0e9 ? (int)f : 2000000000) : 0;
+                c[d] = v % s.nc[d];
+            }
+            sc = s.nat2sorted[(c[2] * s.nc[1] + c[1]) * s.nc[0] + c[0]];
+            s.atom_cell[a] = sc;
         }
-        const int sc = s.nat2sorted[(c[2] * s.nc[1] + c[1]) * s.nc[0] + c[0]];
-        s.atom_cell[a] = sc;
-        atomicAdd(&s.cell_count[sc], 1);
+        int head, len;
+        wave_runs(sc, act, &head, &len);
+        if (act && (int)(threadIdx.x & 63) == head) atomicAdd(&s.cell_count[sc], len);
     }
 }
 
-// Exclusive scan of the cell histogram of one system by one workgroup, in coalesced
-// chunks of 4096 cells (4 per thread); also clears the histogram for the next build.
+// Exclusive scan of the cell histogram of one system by one workgroup: each thread owns a
+// contiguous range of cells (a multiple of 4, read as int4), one block-wide scan of the
+// range sums, then the ranges are written.  The histogram itself returns to zero in
+// k_scatter.
 __global__ __launch_bounds__(1024) void k_scan(const SysDev* __restrict__ sys)
 {
     __shared__ int tmp[16];
     const SysDev& s = sys[blockIdx.x];
     if (!s.ctl[CTL_REBUILD]) return;
     const int n = s.ncell;
-    int carry = 0;
-    for (int base = 0; base < n; base += 4 * (int)blockDim.x) {
-        const int i0 = base + 4 * (int)threadIdx.x;
-        int c[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) c[k] = (i0 + k < n) ? s.cell_count[i0 + k] : 0;
-        int total;
-        int run = carry + block_excl_scan(c[0] + c[1] + c[2] + c[3], tmp, &total);
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (i0 + k < n) {
-                s.cell_start[i0 + k] = run;
-                run += c[k];
-                s.cell_count[i0 + k] = 0;
-                s.cell_fill[i0 + k] = 0;
-            }
-        carry += total;
+    const int per = ((n + 4 * (int)blockDim.x - 1) / (4 * (int)blockDim.x)) * 4;
+    const int i0 = (int)threadIdx.x * per, i1 = min(i0 + per, n);
+    int sum = 0;
+    for (int i = i0; i < i1; i += 4) {
+        if (i + 4 <= i1) {
+            const int4 c = *reinterpret_cast<const int4*>(s.cell_count + i);
+            sum += (c.x + c.y) + (c.z + c.w);
+        } else {
+            for (int k = i; k < i1; ++k) sum += s.cell_count[k];
+        }
     }
-    if (threadIdx.x == 0) s.cell_start[n] = carry;
+    int total;
+    int run = block_excl_scan(sum, tmp, &total);
+    for (int i = i0; i < i1; i += 4) {
+        if (i + 4 <= i1) {
+            const int4 c = *reinterpret_cast<const int4*>(s.cell_count + i);
+            int4 o;
+            o.x = run;
+            o.y = o.x + c.x;
+            o.z = o.y + c.y;
+            o.w = o.z + c.z;
+            run = o.w + c.w;
+            *reinterpret_cast<int4*>(s.cell_start + i) = o;
+        } else {
+            for (int k = i; k < i1; ++k) {
+                s.cell_start[k] = run;
+                run += s.cell_count[k];
+            }
+        }
+    }
+    if (threadIdx.x == 0) s.cell_start[n] = total;
 }
 
+// Each run of same-cell lanes takes a block of slots with one atomicSub; the decrements
+// bring the histogram back to zero for the next rebuild.
 __global__ __launch_bounds__(kAtomThreads) void k_scatter(const SysDev* __restrict__ sys,
                                                            const WorkItem* __restrict__ work, int nwork)
 {
+    if (!block_rebuild_pending<false>(sys, work, nwork)) return;
+    const int lane = threadIdx.x & 63;
     for (int b = blockIdx.x; b < nwork; b += gridDim.x) {
         const WorkItem w = work[b];
         const SysDev& s = sys[w.sys];
         if (!s.ctl[CTL_REBUILD]) continue;
         const int a = w.idx + threadIdx.x;
-        if (a >= s.n) continue;
-        const State st = s.st[__builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR])];
-        const int sc = s.atom_cell[a];
-        const int p = s.cell_start[sc] + atomicAdd(&s.cell_fill[sc], 1);
-        s.scat_src[p] = a;
-        s.scat_key[p] = st.orig[a];
-        s.scat_x[p] = st.x[a];
+        const bool act = a < s.n;
+        const int sc = act ? s.atom_cell[a] : 0;
+        int head, len;
+        wave_runs(sc, act, &head, &len);
+        int old = 0;
+        if (act && lane == head) old = atomicSub(&s.cell_count[sc], len);
+        old = __shfl(old, head, 64);
+        if (act) {
+            const State st = s.st[__builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR])];
+            const int p = s.cell_start[sc] + old - len + (lane - head);
+            s.scat_src[p] = a;
+            s.scat_key[p] = st.orig[a];
+            s.scat_x[p] = st.x[a];
+        }
     }
 }
 
 // Inside a cell, atoms end up in ascending x (ties by original index), whatever order
 // the atomics placed them in: the order is a function of the positions alone, and a row
-// of cells along x is then sorted by x, which the list build's x windows rely on.
+// of cells along x is then sorted by x, which the list build's x windows rely on.  The
+// sort keys of the cells a block's 256 slots fall in are staged in LDS when they fit.
+constexpr int kRankStage = 2048;
+
 __global__ __launch_bounds__(kAtomThreads) void k_rank_permute(const SysDev* __restrict__ sys,
                                                                 const WorkItem* __restrict__ work,
                                                                 int nwork)
 {
+    __shared__ double kx[kRankStage];
+    __shared__ uint32_t kk[kRankStage];
+    __shared__ int range[2];
+    if (!block_rebuild_pending<false>(sys, work, nwork)) return;
     for (int b = blockIdx.x; b < nwork; b += gridDim.x) {
         const WorkItem w = work[b];
         const SysDev& s = sys[w.sys];
-        if (!s.ctl[CTL_REBUILD]) continue;
+        if (!s.ctl[CTL_REBUILD]) continue;  // uniform over the block
         const int p = w.idx + threadIdx.x;
-        if (p >= s.n) continue;
-        const int cur = __builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR]);
-        const State src = s.st[cur], dst = s.st[cur ^ 1];
-        const int a = s.scat_src[p];
-        const int sc = s.atom_cell[a];
-        const int c0 = s.cell_start[sc], c1 = s.cell_start[sc + 1];
-        const uint32_t key = s.scat_key[p];
-        const double x = src.x[a], y = src.y[a], z = src.z[a];
-        int rank = 0;
-        for (int q = c0; q < c1; ++q) {
-            const double xq = s.scat_x[q];
-            const uint32_t kq = s.scat_key[q];
-            rank += (xq < x || (xq == x && kq < key)) ? 1 : 0;
+        const int last = min(w.idx + (int)blockDim.x, s.n) - 1;
+        const bool act = p <= last;
+        int a = 0, sc = 0;
+        if (act) {
+            a = s.scat_src[p];
+            sc = s.atom_cell[a];
         }
-        const int np = c0 + rank;
-        dst.x[np] = x;
-        dst.y[np] = y;
-        dst.z[np] = z;
-        s.rx[np] = x;
-        s.ry[np] = y;
-        s.rz[np] = z;
-        dst.vx[np] = src.vx[a];
-        dst.vy[np] = src.vy[a];
-        dst.vz[np] = src.vz[a];
-        dst.fx[np] = src.fx[a];
-        dst.fy[np] = src.fy[a];
-        dst.fz[np] = src.fz[a];
-        dst.type[np] = src.type[a];
-        dst.orig[np] = key;
+        // slots are grouped by cell, so the block's cells are those of its first and last slot
+        if (threadIdx.x == 0) range[0] = s.cell_start[sc];
+        if (p == last) range[1] = s.cell_start[sc + 1];
+        __syncthreads();
+        const int base = range[0], cnt = range[1] - range[0];
+        const bool staged = cnt <= kRankStage;
+        if (staged)
+            for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+                kx[q] = s.scat_x[base + q];
+                kk[q] = s.scat_key[base + q];
+            }
+        __syncthreads();
+        if (act) {
+            const int cur = __builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR]);
+            const State src = s.st[cur], dst = s.st[cur ^ 1];
+            const int c0 = s.cell_start[sc], c1 = s.cell_start[sc + 1];
+            const uint32_t key = s.scat_key[p];
+            const double x = src.x[a], y = src.y[a], z = src.z[a];
+            int rank = 0;
+            if (staged) {
+                for (int q = c0 - base; q < c1 - base; ++q) {
+                    const double xq = kx[q];
+                    const uint32_t kq = kk[q];
+                    rank += (xq < x || (xq == x && kq < key)) ? 1 : 0;
+                }
+            } else {
+                for (int q = c0; q < c1; ++q) {
+                    const double xq = s.scat_x[q];
+                    const uint32_t kq = s.scat_key[q];
+                    rank += (xq < x || (xq == x && kq < key)) ? 1 : 0;
+                }
+            }
+            const int np = c0 + rank;
+            dst.x[np] = x;
+            dst.y[np] = y;
+            dst.z[np] = z;
+            s.rx[np] = x;
+            s.ry[np] = y;
+            s.rz[np] = z;
+            dst.vx[np] = src.vx[a];
+            dst.vy[np] = src.vy[a];
+            dst.vz[np] = src.vz[a];
+            dst.fx[np] = src.fx[a];
+            dst.fy[np] = src.fy[a];
+            dst.fz[np] = src.fz[a];
+            dst.type[np] = src.type[a];
+            dst.orig[np] = key;
+        }
+        __syncthreads();  // the next item restages the keys
     }
 }
 
@@ -734,6 +837,7 @@ __global__ __launch_bounds__(kTileThreads) void k_build_list(const SysDev* __res
     __shared__ int stmp[16];
     __shared__ PairParam ppl[kMaxTypes * kMaxTypes];
     const int lane = threadIdx.x & 63;
+    if (!block_rebuild_pending<true>(sys, tiles, ntw)) return;
     for (int b = blockIdx.x; b < ntw; b += gridDim.x) {
         const WorkItem w = tiles[xcd_remap(b, ntw)];
         const SysDev& s = sys[w.sys];
@@ -1308,7 +1412,7 @@ void launch_rebuild_sort(hipStream_t st, const SysDev* d_sys, int nsys, const Wo
 {
     // Rebuild kernels loop over their work items, so a modest grid covers any size and
     // costs little on the steps where they return at once.
-    const int ga = grid_cap(n_atom_work, 512);
+    const int ga = grid_cap(n_atom_work, PIS_SORT_GRID_CAP);
     hipLaunchKernelGGL(k_bin, dim3(ga), dim3(kAtomThreads), 0, st, d_sys, atom_work, n_atom_work);
     hipLaunchKernelGGL(k_scan, dim3(nsys), dim3(1024), 0, st, d_sys);
     hipLaunchKernelGGL(k_scatter, dim3(ga), dim3(kAtomThreads), 0, st, d_sys, atom_work, n_atom_work);
