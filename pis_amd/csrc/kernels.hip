@@ -39,7 +39,7 @@
 #define PIS_FORCE_MIN_WAVES_FP64 1
 #endif
 #ifndef PIS_BUILD_UNROLL
-#define PIS_BUILD_UNROLL 4
+#define PIS_BUILD_UNROLL 8
 #endif
 #ifndef PIS_SORT_GRID_CAP
 #define PIS_SORT_GRID_CAP 1024
@@ -52,6 +52,19 @@ namespace pis {
 
 // ---------------------------------------------------------------------------------
 // helpers
+
+// Global-address-space view of a device pointer.  Pointers read out of the system
+// descriptors are generic to the compiler, which then emits FLAT memory instructions; a
+// FLAT access also counts against the LDS/scalar-load wait counter, so the next LDS or
+// scalar wait would wait for it to reach memory.  The cast (the address is unchanged) lets
+// the compiler emit GLOBAL instructions instead.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gbl(T* p)
+{
+    return (__attribute__((address_space(1))) T*)p;
+}
 
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range
 // of tiles so neighbouring tiles (which share halo atoms) hit the same L2.
@@ -211,10 +224,10 @@ __device__ int stage_tile(const SysDev& s, const State& st, const TileGeom& g, d
         const int cxn = wrap_cell(g.x0 - 1 + ex, s.nc[0]);
         const int cyn = wrap_cell(g.y0 - 1 + ey, s.nc[1]);
         const int czn = wrap_cell(g.z0 - 1 + ez, s.nc[2]);
-        const int sc = s.nat2sorted[(czn * s.nc[1] + cyn) * s.nc[0] + cxn];
-        const int b = s.cell_start[sc];
+        const int sc = gbl(s.nat2sorted)[(czn * s.nc[1] + cyn) * s.nc[0] + cxn];
+        const int b = gbl(s.cell_start)[sc];
         ecell[tid] = b;
-        cnt = s.cell_start[sc + 1] - b;
+        cnt = gbl(s.cell_start)[sc + 1] - b;
     }
     int total;
     const int ex = block_excl_scan(cnt, stmp, &total);
@@ -235,7 +248,7 @@ __device__ int stage_tile(const SysDev& s, const State& st, const TileGeom& g, d
         if (WHAT == STAGE_INDEX) {
             reinterpret_cast<int*>(Px)[sl] = a;
         } else {
-            double x = st.x[a], y = st.y[a], z = st.z[a];
+            double x = gbl(st.x)[a], y = gbl(st.y)[a], z = gbl(st.z)[a];
             if (WHAT == STAGE_REL) {
                 x = mi_fast(x - cx, s.L[0], s.Linv[0]);
                 y = mi_fast(y - cy, s.L[1], s.Linv[1]);
@@ -244,7 +257,7 @@ __device__ int stage_tile(const SysDev& s, const State& st, const TileGeom& g, d
             Px[sl] = (T)x;
             Py[sl] = (T)y;
             Pz[sl] = (T)z;
-            Pt[sl] = st.type[a];
+            Pt[sl] = gbl(st.type)[a];
         }
     }
     if (tid == 0 && WHAT != STAGE_INDEX) {
@@ -524,27 +537,27 @@ __global__ __launch_bounds__(kAtomThreads) void k_rank_permute(const SysDev* __r
         const bool act = p <= last;
         int a = 0, sc = 0;
         if (act) {
-            a = s.scat_src[p];
-            sc = s.atom_cell[a];
+            a = gbl(s.scat_src)[p];
+            sc = gbl(s.atom_cell)[a];
         }
         // slots are grouped by cell, so the block's cells are those of its first and last slot
-        if (threadIdx.x == 0) range[0] = s.cell_start[sc];
-        if (p == last) range[1] = s.cell_start[sc + 1];
+        if (threadIdx.x == 0) range[0] = gbl(s.cell_start)[sc];
+        if (p == last) range[1] = gbl(s.cell_start)[sc + 1];
         __syncthreads();
         const int base = range[0], cnt = range[1] - range[0];
         const bool staged = cnt <= kRankStage;
         if (staged)
             for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
-                kx[q] = s.scat_x[base + q];
-                kk[q] = s.scat_key[base + q];
+                kx[q] = gbl(s.scat_x)[base + q];
+                kk[q] = gbl(s.scat_key)[base + q];
             }
         __syncthreads();
         if (act) {
             const int cur = __builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR]);
             const State src = s.st[cur], dst = s.st[cur ^ 1];
-            const int c0 = s.cell_start[sc], c1 = s.cell_start[sc + 1];
-            const uint32_t key = s.scat_key[p];
-            const double x = src.x[a], y = src.y[a], z = src.z[a];
+            const int c0 = gbl(s.cell_start)[sc], c1 = gbl(s.cell_start)[sc + 1];
+            const uint32_t key = gbl(s.scat_key)[p];
+            const double x = gbl(src.x)[a], y = gbl(src.y)[a], z = gbl(src.z)[a];
             int rank = 0;
             if (staged) {
                 for (int q = c0 - base; q < c1 - base; ++q) {
@@ -554,26 +567,26 @@ __global__ __launch_bounds__(kAtomThreads) void k_rank_permute(const SysDev* __r
                 }
             } else {
                 for (int q = c0; q < c1; ++q) {
-                    const double xq = s.scat_x[q];
-                    const uint32_t kq = s.scat_key[q];
+                    const double xq = gbl(s.scat_x)[q];
+                    const uint32_t kq = gbl(s.scat_key)[q];
                     rank += (xq < x || (xq == x && kq < key)) ? 1 : 0;
                 }
             }
             const int np = c0 + rank;
-            dst.x[np] = x;
-            dst.y[np] = y;
-            dst.z[np] = z;
-            s.rx[np] = x;
-            s.ry[np] = y;
-            s.rz[np] = z;
-            dst.vx[np] = src.vx[a];
-            dst.vy[np] = src.vy[a];
-            dst.vz[np] = src.vz[a];
-            dst.fx[np] = src.fx[a];
-            dst.fy[np] = src.fy[a];
-            dst.fz[np] = src.fz[a];
-            dst.type[np] = src.type[a];
-            dst.orig[np] = key;
+            gbl(dst.x)[np] = x;
+            gbl(dst.y)[np] = y;
+            gbl(dst.z)[np] = z;
+            gbl(s.rx)[np] = x;
+            gbl(s.ry)[np] = y;
+            gbl(s.rz)[np] = z;
+            gbl(dst.vx)[np] = gbl(src.vx)[a];
+            gbl(dst.vy)[np] = gbl(src.vy)[a];
+            gbl(dst.vz)[np] = gbl(src.vz)[a];
+            gbl(dst.fx)[np] = gbl(src.fx)[a];
+            gbl(dst.fy)[np] = gbl(src.fy)[a];
+            gbl(dst.fz)[np] = gbl(src.fz)[a];
+            gbl(dst.type)[np] = gbl(src.type)[a];
+            gbl(dst.orig)[np] = key;
         }
         __syncthreads();  // the next item restages the keys
     }
@@ -599,31 +612,28 @@ __device__ __forceinline__ int slot_atom(const int* pref, const int* ecell, int 
 __device__ __forceinline__ bool exact_within(const SysDev& s, const State& st, int a, double xi,
                                              double yi, double zi, double r2max)
 {
-    const double dx = mi_ref(st.x[a] - xi, s.L[0], s.Linv[0]);
-    const double dy = mi_ref(st.y[a] - yi, s.L[1], s.Linv[1]);
-    const double dz = mi_ref(st.z[a] - zi, s.L[2], s.Linv[2]);
+    const double dx = mi_ref(gbl(st.x)[a] - xi, s.L[0], s.Linv[0]);
+    const double dy = mi_ref(gbl(st.y)[a] - yi, s.L[1], s.Linv[1]);
+    const double dz = mi_ref(gbl(st.z)[a] - zi, s.L[2], s.Linv[2]);
     return (dx * dx + dy * dy) + dz * dz <= r2max;
 }
 
-// Full neighbour list at rcut_ij + skin with the predicate of build_neighbour_list
-// (lennard_jones.rs:398-405) decided exactly.
-//
-// A wavefront owns one cell of the tile at a time and walks its atoms one by one.  For
-// each atom the 64 lanes sweep the cell's stencil -- the 9 contiguous (dy,dz) rows of the
-// staging -- 64 consecutive slots at a time (conflict-free LDS reads), so control flow is
-// uniform and the cutoff compare is directly the ballot of accepted candidates.  Accepted
-// lanes compact their entries into a per-wave LDS list at count + (accepted lanes below),
-// and the finished list leaves in whole uint4 groups, one store instruction per atom.
-// r2 from the staged coordinates decides every pair outside a +-band around the cutoff;
-// inside the band the bit-exact reference predicate (exact_within, raw positions) decides.
-// Entries are LDS byte offsets.
+// Maximum of v over the wave (all lanes active), uniform: DPP within each row of 16 lanes
+// (quad swaps, half-row and row mirrors), then the four row maxima through scalar
+// registers.  No LDS round trips, unlike a shuffle reduction.
 __device__ __forceinline__ int wave_max(int v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));  // row_mirror
+    const int r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const int r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return max(max(r0, r1), max(r2, r3));
 }
 
+// Full neighbour list at rcut_ij + skin, accepted by the predicate of build_neighbour_list
+// (lennard_jones.rs:398-405) decided exactly; entries are LDS byte offsets.
 // Neighbour list of the own atom held by this lane (lane per atom).  The lane sweeps its
 // cell's stencil -- 9 contiguous (dy,dz) rows of the staging -- in chunks of up to 64
 // candidates with a trip count that is uniform over the wave (the wave's longest chunk),
@@ -642,10 +652,10 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
     double xi = 0.0, yi = 0.0, zi = 0.0;
     int ti = 0, e0 = 0, self = -1, ly = 0, lz = 0;
     if (active) {
-        xi = st.x[a];
-        yi = st.y[a];
-        zi = st.z[a];
-        ti = st.type[a];
+        xi = gbl(st.x)[a];
+        yi = gbl(st.y)[a];
+        zi = gbl(st.z)[a];
+        ti = gbl(st.type)[a];
         if (s.mode == MODE_WHOLE)
             self = a;
         else
@@ -682,7 +692,8 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
     uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;  // 8 list entries of 16 bits, w0 oldest
     int cnt = 0;
     const int maxg = s.maxnb >> 3;
-    uint4* out = s.nbr + a;
+    const size_t npad = s.npad;
+    __attribute__((address_space(1))) u32x4* out = gbl(reinterpret_cast<u32x4*>(s.nbr + a));
     const int nrows = s.mode == MODE_WHOLE ? 1 : 9;
     const int hx = g.sx + 2, hxy = (g.sx + 2) * (g.sy + 2);
     for (int r = 0; r < nrows; ++r) {
@@ -711,51 +722,59 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
             }
         }
         // uniform over the wave by construction; readfirstlane makes the loops scalar
-        const int rowmax = __builtin_amdgcn_readfirstlane(wave_max(re - rb));
+        const int rowmax = wave_max(re - rb);
         for (int cb = 0; cb < rowmax; cb += 64) {
             constexpr int U = PIS_BUILD_UNROLL;
             // trips of U candidates up to the wave's longest window (<= 64); a lane's bits
-            // past its own window, and its own atom, are cleared once per chunk below
+            // past its own window, and its own atom, are cleared once per chunk below.  A
+            // lane whose window is used up reads from slot 0; the others read at most 63
+            // slots past their chunk start, inside the staging's padding.
             const int kend = (min(64, rowmax - cb) + U - 1) & ~(U - 1);
             const int sb = rb + cb;
+            const int sr = sb < re ? sb : 0;
             uint32_t mlo = 0, mhi = 0, blo_m = 0, bhi_m = 0;
-            for (int k = 0; k < kend; k += U) {
-                float dx[U], dy[U], dz[U];
-                int tt[U];
+            // two halves of 32 candidates: bit positions are scalar, no branch per candidate
+            for (int half = 0; half < 2; ++half) {
+                const int k0 = 32 * half, k1 = min(kend, k0 + 32);
+                uint32_t mi = 0, mm = 0;  // certainly within / within the band's upper edge
+                for (int k = k0; k < k1; k += U) {
+                    float dx[U], dy[U], dz[U];
+                    int tt[U];
 #pragma unroll
-                for (int j = 0; j < U; ++j) {
-                    const int sli = min(sb + k + j, SCAP);  // stays inside the staging
-                    dx[j] = Px[sli] - qx32;
-                    dy[j] = Py[sli] - qy32;
-                    dz[j] = Pz[sli] - qz32;
-                    tt[j] = MULTI ? (int)Pt[sli] : 0;
+                    for (int j = 0; j < U; ++j) {
+                        const int sli = sr + k + j;
+                        dx[j] = Px[sli] - qx32;
+                        dy[j] = Py[sli] - qy32;
+                        dz[j] = Pz[sli] - qz32;
+                        tt[j] = MULTI ? (int)Pt[sli] : 0;
+                    }
+#pragma unroll
+                    for (int j = 0; j < U; ++j) {
+                        if (PAIRMI) {
+                            dx[j] = mi_fast32(dx[j], L32[0], Li32[0]);
+                            dy[j] = mi_fast32(dy[j], L32[1], Li32[1]);
+                            dz[j] = mi_fast32(dz[j], L32[2], Li32[2]);
+                        }
+                        const float r2 = fmaf(dx[j], dx[j], fmaf(dy[j], dy[j], dz[j] * dz[j]));
+                        float blo = tlo[0], bhi = thi[0];
+                        if (MULTI) {
+                            blo = tt[j] == 1 ? tlo[1] : blo;
+                            bhi = tt[j] == 1 ? thi[1] : bhi;
+                            blo = tt[j] == 2 ? tlo[2] : blo;
+                            bhi = tt[j] == 2 ? thi[2] : bhi;
+                            blo = tt[j] == 3 ? tlo[3] : blo;
+                            bhi = tt[j] == 3 ? thi[3] : bhi;
+                        }
+                        mi |= (r2 <= blo ? 1u : 0u) << (k - k0 + j);
+                        mm |= (r2 <= bhi ? 1u : 0u) << (k - k0 + j);
+                    }
                 }
-#pragma unroll
-                for (int j = 0; j < U; ++j) {
-                    if (PAIRMI) {
-                        dx[j] = mi_fast32(dx[j], L32[0], Li32[0]);
-                        dy[j] = mi_fast32(dy[j], L32[1], Li32[1]);
-                        dz[j] = mi_fast32(dz[j], L32[2], Li32[2]);
-                    }
-                    const float r2 = fmaf(dx[j], dx[j], fmaf(dy[j], dy[j], dz[j] * dz[j]));
-                    float blo = tlo[0], bhi = thi[0];
-                    if (MULTI) {
-                        blo = tt[j] == 1 ? tlo[1] : blo;
-                        bhi = tt[j] == 1 ? thi[1] : bhi;
-                        blo = tt[j] == 2 ? tlo[2] : blo;
-                        bhi = tt[j] == 2 ? thi[2] : bhi;
-                        blo = tt[j] == 3 ? tlo[3] : blo;
-                        bhi = tt[j] == 3 ? thi[3] : bhi;
-                    }
-                    const uint32_t in = r2 <= blo ? 1u : 0u;
-                    const uint32_t bd = (r2 > blo && r2 <= bhi) ? 1u : 0u;
-                    if (k + j < 32) {
-                        mlo |= in << (k + j);
-                        blo_m |= bd << (k + j);
-                    } else {
-                        mhi |= in << (k + j - 32);
-                        bhi_m |= bd << (k + j - 32);
-                    }
+                if (half == 0) {
+                    mlo = mi;
+                    blo_m = mm & ~mi;
+                } else {
+                    mhi = mi;
+                    bhi_m = mm & ~mi;
                 }
             }
             {
@@ -803,7 +822,7 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
                     w3 = __builtin_amdgcn_alignbit((base + b) << 3, w3, 16);
                     ++cnt;
                     if ((cnt & 7) == 0 && (cnt >> 3) <= maxg)
-                        out[(size_t)((cnt >> 3) - 1) * s.npad] = make_uint4(w0, w1, w2, w3);
+                        out[(size_t)((cnt >> 3) - 1) * npad] = u32x4{w0, w1, w2, w3};
                 }
             }
         }
@@ -818,9 +837,9 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
             w3 = __builtin_amdgcn_alignbit((uint32_t)(SCAP << 3), w3, 16);
             ++cnt;
         }
-        if ((cnt >> 3) <= maxg) out[(size_t)((cnt >> 3) - 1) * s.npad] = make_uint4(w0, w1, w2, w3);
+        if ((cnt >> 3) <= maxg) out[(size_t)((cnt >> 3) - 1) * npad] = u32x4{w0, w1, w2, w3};
     }
-    s.nnb[a] = (uint16_t)min(n_real, s.maxnb);
+    gbl(s.nnb)[a] = (uint16_t)min(n_real, s.maxnb);
     return n_real;
 }
 
@@ -829,9 +848,10 @@ __global__ __launch_bounds__(kTileThreads) void k_build_list(const SysDev* __res
                                                               const WorkItem* __restrict__ tiles,
                                                               int ntw)
 {
-    // FP32 staging: half the LDS of the force kernel's, so more tiles share a CU
-    __shared__ float Px[SCAP + 1], Py[SCAP + 1], Pz[SCAP + 1];
-    __shared__ uint8_t Pt[SCAP + 1];
+    // FP32 staging: half the LDS of the force kernel's, so more tiles share a CU.  The 64
+    // slots past the padding slot take the candidate reads that run past a window's end.
+    __shared__ float Px[SCAP + 1 + 64], Py[SCAP + 1 + 64], Pz[SCAP + 1 + 64];
+    __shared__ uint8_t Pt[SCAP + 1 + 64];
     __shared__ int pref[kMaxHalo + 1];
     __shared__ int ecell[kMaxHalo];
     __shared__ int stmp[16];
@@ -940,7 +960,6 @@ struct ExactCtx {
     double xi, yi, zi;
 };
 
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const u32x4 gu32x4;
 
 template <int SCAP, bool MIXED, bool MULTI, bool PAIRMI>
@@ -950,7 +969,7 @@ __device__ __forceinline__ void atom_forces(const SysDev& s, const double* Px, c
                                             double qx, double qy, double qz, const ExactCtx& ex,
                                             Acc& acc)
 {
-    const int nn = s.nnb[a];
+    const int nn = gbl(s.nnb)[a];
     const int ng = (nn + 7) >> 3;
     const gu32x4* L = (const gu32x4*)(s.nbr + a);
     const char* bx = reinterpret_cast<const char*>(Px);
@@ -1065,10 +1084,10 @@ __global__ __launch_bounds__(kTileThreads, MIXED ? PIS_FORCE_MIN_WAVES_MIXED : P
     double sum[4] = {0.0, 0.0, 0.0, 0.0};
     if (total <= SCAP) {
         const PairParam p0 = s.pp[0];  // uniform: scalar loads, no vector registers
-        const int a0 = s.cell_start[g.c0], a1 = s.cell_start[g.c1];
+        const int a0 = gbl(s.cell_start)[g.c0], a1 = gbl(s.cell_start)[g.c1];
         for (int a = a0 + (int)threadIdx.x; a < a1; a += blockDim.x) {
-            const double xi = st.x[a], yi = st.y[a], zi = st.z[a];
-            const int ti = st.type[a];
+            const double xi = gbl(st.x)[a], yi = gbl(st.y)[a], zi = gbl(st.z)[a];
+            const int ti = gbl(st.type)[a];
             double qx = xi, qy = yi, qz = zi;
             if (rel) {
                 qx = mi_fast(xi - cx, s.L[0], s.Linv[0]);
@@ -1090,29 +1109,29 @@ __global__ __launch_bounds__(kTileThreads, MIXED ? PIS_FORCE_MIN_WAVES_MIXED : P
             }
             const double m = s.mass[ti];
             // positions again from memory (L2): not held in registers across the pair loop
-            const double xa = st.x[a], ya = st.y[a], za = st.z[a];
-            double vx = st.vx[a], vy = st.vy[a], vz = st.vz[a];
+            const double xa = gbl(st.x)[a], ya = gbl(st.y)[a], za = gbl(st.z)[a];
+            double vx = gbl(st.vx)[a], vy = gbl(st.vy)[a], vz = gbl(st.vz)[a];
             if (integrate) {
                 // v += ((a_t + a_{t+dt}) 0.5) dt  (potential.rs:28-30)
                 const double dt = s.dt;
-                vx = vx + ((st.fx[a] / m + acc.fx / m) * 0.5) * dt;
-                vy = vy + ((st.fy[a] / m + acc.fy / m) * 0.5) * dt;
-                vz = vz + ((st.fz[a] / m + acc.fz / m) * 0.5) * dt;
+                vx = vx + ((gbl(st.fx)[a] / m + acc.fx / m) * 0.5) * dt;
+                vy = vy + ((gbl(st.fy)[a] / m + acc.fy / m) * 0.5) * dt;
+                vz = vz + ((gbl(st.fz)[a] / m + acc.fz / m) * 0.5) * dt;
                 if (s.nhc_on) {  // NVT: the second scaling (potential.rs:49)
                     const double sc = s.nhc[NHC_SCALE];
                     vx = vx * sc;
                     vy = vy * sc;
                     vz = vz * sc;
                 }
-                wt.vx[a] = vx;
-                wt.vy[a] = vy;
-                wt.vz[a] = vz;
-                wt.fx[a] = acc.fx;
-                wt.fy[a] = acc.fy;
-                wt.fz[a] = acc.fz;
+                gbl(wt.vx)[a] = vx;
+                gbl(wt.vy)[a] = vy;
+                gbl(wt.vz)[a] = vz;
+                gbl(wt.fx)[a] = acc.fx;
+                gbl(wt.fy)[a] = acc.fy;
+                gbl(wt.fz)[a] = acc.fz;
                 if (rebuilt) {  // the other set still has the order before the sort
-                    wt.type[a] = (uint8_t)ti;
-                    wt.orig[a] = st.orig[a];
+                    gbl(wt.type)[a] = (uint8_t)ti;
+                    gbl(wt.orig)[a] = gbl(st.orig)[a];
                 }
                 if (ahead) {
                     // the next step's x += v dt + (F/m)/2 dt^2, wrap and skin check, as
@@ -1128,26 +1147,26 @@ __global__ __launch_bounds__(kTileThreads, MIXED ? PIS_FORCE_MIN_WAVES_MIXED : P
                     x = s.L[0] * sx;
                     y = s.L[1] * sy;
                     z = s.L[2] * sz;
-                    wt.x[a] = x;
-                    wt.y[a] = y;
-                    wt.z[a] = z;
-                    const double ddx = mi_fast(x - s.rx[a], s.L[0], s.Linv[0]);
-                    const double ddy = mi_fast(y - s.ry[a], s.L[1], s.Linv[1]);
-                    const double ddz = mi_fast(z - s.rz[a], s.L[2], s.Linv[2]);
+                    gbl(wt.x)[a] = x;
+                    gbl(wt.y)[a] = y;
+                    gbl(wt.z)[a] = z;
+                    const double ddx = mi_fast(x - gbl(s.rx)[a], s.L[0], s.Linv[0]);
+                    const double ddy = mi_fast(y - gbl(s.ry)[a], s.L[1], s.Linv[1]);
+                    const double ddz = mi_fast(z - gbl(s.rz)[a], s.L[2], s.Linv[2]);
                     const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
                     if (!(d2 <= s.half_skin2)) {  // also catches NaN
                         atomicOr(&s.ctl[CTL_REBUILD_NEXT], 1);
                         if (d2 != d2) atomicOr(&s.ctl[CTL_NAN], 1);
                     }
                 } else {
-                    wt.x[a] = xa;
-                    wt.y[a] = ya;
-                    wt.z[a] = za;
+                    gbl(wt.x)[a] = xa;
+                    gbl(wt.y)[a] = ya;
+                    gbl(wt.z)[a] = za;
                 }
             } else {
-                st.fx[a] = acc.fx;
-                st.fy[a] = acc.fy;
-                st.fz[a] = acc.fz;
+                gbl(st.fx)[a] = acc.fx;
+                gbl(st.fy)[a] = acc.fy;
+                gbl(st.fz)[a] = acc.fz;
             }
             sum[0] += acc.pe;
             sum[1] += 0.5 * m * ((vx * vx + vy * vy) + vz * vz);
