@@ -370,23 +370,23 @@ __global__ __launch_bounds__(kAtomThreads) void k_integrate_pos(const SysDev* __
     const int a = w.idx + threadIdx.x;
     if (a >= s.n || s.ctl[CTL_LEFT] == 0) return;
     const State st = s.st[__builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR])];
-    const double m = s.mass[st.type[a]];
+    const double m = gbl(s.mass)[gbl(st.type)[a]];
     const double dt = s.dt, dt2 = s.dt2;
-    double vx = st.vx[a], vy = st.vy[a], vz = st.vz[a];
+    double vx = gbl(st.vx)[a], vy = gbl(st.vy)[a], vz = gbl(st.vz)[a];
     if (s.nhc_on) {
         // NVT: v *= exp(-dt xi0 / 2) before the NVE step (potential.rs:44-45)
         const double sc = s.nhc[NHC_SCALE];
         vx = vx * sc;
         vy = vy * sc;
         vz = vz * sc;
-        st.vx[a] = vx;
-        st.vy[a] = vy;
-        st.vz[a] = vz;
+        gbl(st.vx)[a] = vx;
+        gbl(st.vy)[a] = vy;
+        gbl(st.vz)[a] = vz;
     }
     // a_t = F/m (current_acceleration, properties.rs:32-39); x += v dt + (a 0.5) dt^2
-    double x = st.x[a] + (vx * dt + (st.fx[a] / m) * 0.5 * dt2);
-    double y = st.y[a] + (vy * dt + (st.fy[a] / m) * 0.5 * dt2);
-    double z = st.z[a] + (vz * dt + (st.fz[a] / m) * 0.5 * dt2);
+    double x = gbl(st.x)[a] + (vx * dt + (gbl(st.fx)[a] / m) * 0.5 * dt2);
+    double y = gbl(st.y)[a] + (vy * dt + (gbl(st.fy)[a] / m) * 0.5 * dt2);
+    double z = gbl(st.z)[a] + (vz * dt + (gbl(st.fz)[a] / m) * 0.5 * dt2);
     // apply_boundary_conditions_pos (simulation_box.rs:29-42): s = h^-1 r; s -= floor(s); r = h s
     double sx = x * s.Linv[0], sy = y * s.Linv[1], sz = z * s.Linv[2];
     sx = sx - floor(sx);
@@ -395,12 +395,12 @@ __global__ __launch_bounds__(kAtomThreads) void k_integrate_pos(const SysDev* __
     x = s.L[0] * sx;
     y = s.L[1] * sy;
     z = s.L[2] * sz;
-    st.x[a] = x;
-    st.y[a] = y;
-    st.z[a] = z;
-    const double dx = mi_fast(x - s.rx[a], s.L[0], s.Linv[0]);
-    const double dy = mi_fast(y - s.ry[a], s.L[1], s.Linv[1]);
-    const double dz = mi_fast(z - s.rz[a], s.L[2], s.Linv[2]);
+    gbl(st.x)[a] = x;
+    gbl(st.y)[a] = y;
+    gbl(st.z)[a] = z;
+    const double dx = mi_fast(x - gbl(s.rx)[a], s.L[0], s.Linv[0]);
+    const double dy = mi_fast(y - gbl(s.ry)[a], s.L[1], s.Linv[1]);
+    const double dz = mi_fast(z - gbl(s.rz)[a], s.L[2], s.Linv[2]);
     const double d2 = dx * dx + dy * dy + dz * dz;
     if (!(d2 <= s.half_skin2)) {  // also catches NaN
         atomicOr(&s.ctl[CTL_REBUILD], 1);
@@ -425,7 +425,7 @@ __global__ __launch_bounds__(kAtomThreads) void k_bin(const SysDev* __restrict__
         if (act) {
             const State st = s.st[__builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR])];
             // rcut_cells (neighbour_list.rs:17-34): c = floor(s n) as usize (saturating), % n
-            const double r[3] = {st.x[a], st.y[a], st.z[a]};
+            const double r[3] = {gbl(st.x)[a], gbl(st.y)[a], gbl(st.z)[a]};
             int c[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
@@ -433,8 +433,8 @@ __global__ __launch_bounds__(kAtomThreads) void k_bin(const SysDev* __restrict__
                 const int v = f > 0.0 ? (f < 2.0e9 ? (int)f : 2000000000) : 0;
                 c[d] = v % s.nc[d];
             }
-            sc = s.nat2sorted[(c[2] * s.nc[1] + c[1]) * s.nc[0] + c[0]];
-            s.atom_cell[a] = sc;
+            sc = gbl(s.nat2sorted)[(c[2] * s.nc[1] + c[1]) * s.nc[0] + c[0]];
+            gbl(s.atom_cell)[a] = sc;
         }
         int head, len;
         wave_runs(sc, act, &head, &len);
@@ -498,7 +498,7 @@ __global__ __launch_bounds__(kAtomThreads) void k_scatter(const SysDev* __restri
         if (!s.ctl[CTL_REBUILD]) continue;
         const int a = w.idx + threadIdx.x;
         const bool act = a < s.n;
-        const int sc = act ? s.atom_cell[a] : 0;
+        const int sc = act ? gbl(s.atom_cell)[a] : 0;
         int head, len;
         wave_runs(sc, act, &head, &len);
         int old = 0;
@@ -506,10 +506,10 @@ __global__ __launch_bounds__(kAtomThreads) void k_scatter(const SysDev* __restri
         old = __shfl(old, head, 64);
         if (act) {
             const State st = s.st[__builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR])];
-            const int p = s.cell_start[sc] + old - len + (lane - head);
-            s.scat_src[p] = a;
-            s.scat_key[p] = st.orig[a];
-            s.scat_x[p] = st.x[a];
+            const int p = gbl(s.cell_start)[sc] + old - len + (lane - head);
+            gbl(s.scat_src)[p] = a;
+            gbl(s.scat_key)[p] = gbl(st.orig)[a];
+            gbl(s.scat_x)[p] = gbl(st.x)[a];
         }
     }
 }
@@ -1216,13 +1216,12 @@ __device__ double nhc_kinetic(const double* c)
 __device__ __forceinline__ void sum_partials(const SysDev& s, double* v, double* rtmp)
 {
     v[0] = v[1] = v[2] = v[3] = 0.0;
-    const double4* P4 = reinterpret_cast<const double4*>(s.partial);
+    const __attribute__((address_space(1))) double* P = gbl(static_cast<const double*>(s.partial));
     for (int t = threadIdx.x; t < s.ntile; t += blockDim.x) {
-        const double4 q = P4[t];
-        v[0] += q.x;
-        v[1] += q.y;
-        v[2] += q.z;
-        v[3] += q.w;
+        v[0] += P[4 * t + 0];
+        v[1] += P[4 * t + 1];
+        v[2] += P[4 * t + 2];
+        v[3] += P[4 * t + 3];
     }
     block_sum4(v, rtmp);
 }
@@ -1274,36 +1273,39 @@ __global__ __launch_bounds__(1024) void k_reduce(const SysDev* __restrict__ sys,
         c[NHC_T] = c[NHC_T0] + ((c[NHC_T1] - c[NHC_T0]) / c[NHC_TOTAL]) * c[NHC_STEP];
         c[NHC_STEP] = c[NHC_STEP] + 1.0;
     }
+    // control words: all read first, then written (one round trip, not one per word)
+    __attribute__((address_space(1))) int32_t* ctl = gbl(s.ctl);
+    const int c_cur = ctl[CTL_CUR], c_rb = ctl[CTL_REBUILD], c_nrb = ctl[CTL_NREBUILD];
+    const int c_step = ctl[CTL_STEP], c_next = ctl[CTL_REBUILD_NEXT];
+    const int c_remain = ctl[CTL_REMAIN], c_left = ctl[CTL_LEFT];
     if (record) {
-        const int k = s.ctl[CTL_STEP];
-        if (k < s.thermo_cap) {
-            ThermoDev t;
-            t.pe = pe;
-            t.ke = ke;
-            t.etot = ham;
-            t.temperature = (2.0 * ke) / ((double)(3 * (int64_t)s.n) * s.kb);
-            t.pressure = (2.0 * ke + wabs) / (3.0 * s.volume);
-            t.virial_abs = wabs;
-            t.virial_pair = wpair;
-            s.thermo[k] = t;
+        if (c_step < s.thermo_cap) {
+            __attribute__((address_space(1))) double* T = gbl(reinterpret_cast<double*>(s.thermo + c_step));
+            T[0] = pe;
+            T[1] = ke;
+            T[2] = ham;
+            T[3] = (2.0 * ke) / ((double)(3 * (int64_t)s.n) * s.kb);  // temperature
+            T[4] = (2.0 * ke + wabs) / (3.0 * s.volume);             // pressure
+            T[5] = wabs;
+            T[6] = wpair;
         }
-        s.ctl[CTL_STEP] = k + 1;
+        ctl[CTL_STEP] = c_step + 1;
     }
-    if (!isfinite(pe) || !isfinite(ke)) s.ctl[CTL_NAN] = 1;
+    if (!isfinite(pe) || !isfinite(ke)) ctl[CTL_NAN] = 1;
     if (step) {
         // the step wrote into the set it did not read: that set becomes live; the skin
         // check of positions advanced ahead decides the next step's rebuild
-        const int rb = s.ctl[CTL_REBUILD] ? 1 : 0;
-        s.ctl[CTL_CUR] = s.ctl[CTL_CUR] ^ rb ^ 1;
-        s.ctl[CTL_NREBUILD] += rb;
-        s.ctl[CTL_REBUILD] = s.ctl[CTL_REBUILD_NEXT];
-        s.ctl[CTL_REBUILD_NEXT] = 0;
-        if (s.ctl[CTL_REMAIN] > 0) s.ctl[CTL_REMAIN] -= 1;
-        if (s.ctl[CTL_LEFT] > 0) s.ctl[CTL_LEFT] -= 1;
-    } else if (flip && s.ctl[CTL_REBUILD]) {
-        s.ctl[CTL_CUR] ^= 1;
-        s.ctl[CTL_REBUILD] = 0;
-        s.ctl[CTL_NREBUILD] += 1;
+        const int rb = c_rb ? 1 : 0;
+        ctl[CTL_CUR] = c_cur ^ rb ^ 1;
+        ctl[CTL_NREBUILD] = c_nrb + rb;
+        ctl[CTL_REBUILD] = c_next;
+        ctl[CTL_REBUILD_NEXT] = 0;
+        if (c_remain > 0) ctl[CTL_REMAIN] = c_remain - 1;
+        if (c_left > 0) ctl[CTL_LEFT] = c_left - 1;
+    } else if (flip && c_rb) {
+        ctl[CTL_CUR] = c_cur ^ 1;
+        ctl[CTL_REBUILD] = 0;
+        ctl[CTL_NREBUILD] = c_nrb + 1;
     }
 }
 
