@@ -219,12 +219,14 @@ class Atoms:
     arrays, i.e. the reference's column-major 3xN memory."""
 
     def __init__(self, type_ids, masses, positions, velocities=None, sim_box: SimulationBox | None = None):
-        self.type_ids = np.ascontiguousarray(type_ids, dtype=np.uint32)
-        self.masses = np.ascontiguousarray(masses, dtype=np.float64)
-        self.positions = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        # Atoms::new takes ownership of its arrays: copies here, so stepping these atoms
+        # never writes into the caller's arrays
+        self.type_ids = np.array(type_ids, dtype=np.uint32)
+        self.masses = np.array(masses, dtype=np.float64)
+        self.positions = np.array(positions, dtype=np.float64).reshape(-1, 3)
         self.n_atoms = self.positions.shape[0]
         self.velocities = (np.zeros_like(self.positions) if velocities is None
-                           else np.ascontiguousarray(velocities, dtype=np.float64).reshape(-1, 3))
+                           else np.array(velocities, dtype=np.float64).reshape(-1, 3))
         self.forces = np.zeros_like(self.positions)
         self.sim_box = sim_box
 

@@ -146,3 +146,16 @@ def test_server_needs_a_device_and_valid_arguments():
     assert lib.pis_server_create(1 << 30, 4, 10, 7, C.byref(srv)) == 5  # unknown precision
     if lib.pis_device_count(C.byref(C.c_int32())) != 0:
         assert lib.pis_server_create(1 << 30, 4, 10, 1, C.byref(srv)) == 9
+
+
+def test_atoms_own_their_arrays():
+    # Atoms::new takes ownership: stepping the mirror must never write into the caller's
+    # arrays (a workload spec reused afterwards, e.g. as the CPU reference's input)
+    from pis_amd import Atoms
+
+    spec = W.lj_box(64, 0.8, 1.0, seed=1)
+    pos0, vel0 = spec.positions.copy(), spec.velocities.copy()
+    atoms = Atoms(spec.type_ids, spec.masses, spec.positions, spec.velocities, spec.box)
+    atoms.positions += 1.0
+    atoms.velocities *= 2.0
+    assert np.array_equal(spec.positions, pos0) and np.array_equal(spec.velocities, vel0)
