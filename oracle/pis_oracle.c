@@ -667,3 +667,198 @@ int orc_run_nve(orc_sys* s, double dt, int64_t nsteps, int32_t manager, double k
     }
     return 0;
 }
+
+/* ---- MTK barostat (ensemble/npt.rs) and the NPT step --------------------------- */
+
+#define ORC_KB_KJPERMOLEKELVIN 0.0083144621 /* constants.rs:3 */
+
+void orc_mtk_new(orc_mtk* m, double p0, double tau, int64_t n, double t_target)
+{
+    memset(m, 0, sizeof(*m));
+    for (int k = 0; k < 3; ++k) M(m->target, k, k) = p0; /* Matrix3::identity() * p0 */
+    m->w = ((double)(3 * n)) * ORC_KB_KJPERMOLEKELVIN * t_target * (tau * tau);
+}
+
+/* velocities * velocities^T: (r, c) = sum_i v_i[r] v_i[c], ascending i (the gemm order of
+ * matrixmultiply is not restated). */
+void orc_kinetic_tensor(const orc_sys* s, double* out)
+{
+    for (int k = 0; k < 9; ++k) out[k] = 0.0;
+    for (int64_t i = 0; i < s->n; ++i)
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) M(out, r, c) += s->vel[3 * i + r] * s->vel[3 * i + c];
+}
+
+/* positions * forces^T: (r, c) = sum_i x_i[r] F_i[c]. */
+void orc_virial_tensor(const orc_sys* s, double* out)
+{
+    for (int k = 0; k < 9; ++k) out[k] = 0.0;
+    for (int64_t i = 0; i < s->n; ++i)
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) M(out, r, c) += s->pos[3 * i + r] * s->force[3 * i + c];
+}
+
+void orc_pressure_tensor(const orc_sys* s, double* out)
+{
+    double kt[9], wt[9];
+    orc_kinetic_tensor(s, kt);
+    orc_virial_tensor(s, wt);
+    const double v = orc_box_volume(&s->box);
+    for (int k = 0; k < 9; ++k) out[k] = (kt[k] + wt[k]) / v;
+}
+
+/* math.rs:39-41: (a + a^T) * 0.5. */
+static void symmetrize(const double* a, double* out)
+{
+    double t[9];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) M(t, r, c) = (M(a, r, c) + M(a, c, r)) * 0.5;
+    memcpy(out, t, sizeof(t));
+}
+
+void orc_mtk_delta_momentum(const orc_mtk* m, const orc_sys* s, double dt, double* out)
+{
+    double p[9];
+    orc_pressure_tensor(s, p);
+    const double f = orc_box_volume(&s->box) * 0.5 * dt;
+    double d[9];
+    for (int k = 0; k < 9; ++k) d[k] = (p[k] - m->target[k]) * f;
+    symmetrize(d, out);
+}
+
+/* Matrix3 * Matrix3, column by column as gemv: out[:,c] = a * b[:,c]. */
+void orc_mat3_mul(const double* a, const double* b, double* out)
+{
+    double t[9];
+    for (int c = 0; c < 3; ++c) orc_gemv(a, &b[3 * c], &t[3 * c]);
+    memcpy(out, t, sizeof(t));
+}
+
+/* exp(A) by scaling and squaring a Taylor series (Moler & Van Loan, "Nineteen dubious
+ * ways to compute the exponential of a matrix", 2003, method 3): A is halved until its
+ * 1-norm is at most 1/2, E = sum_{k<=18} A^k/k!, then squared back. */
+void orc_mat3_exp(const double* a, double* out)
+{
+    double norm = 0.0;
+    for (int c = 0; c < 3; ++c) {
+        const double col = (fabs(M(a, 0, c)) + fabs(M(a, 1, c))) + fabs(M(a, 2, c));
+        norm = col > norm ? col : norm;
+    }
+    int sq = 0;
+    double scale = 1.0;
+    while (norm * scale > 0.5 && sq < 64) {
+        scale *= 0.5;
+        ++sq;
+    }
+    double x[9], e[9], t[9];
+    for (int k = 0; k < 9; ++k) {
+        x[k] = a[k] * scale;
+        e[k] = 0.0;
+        t[k] = 0.0;
+    }
+    for (int k = 0; k < 3; ++k) M(e, k, k) = M(t, k, k) = 1.0;
+    for (int j = 1; j <= 18; ++j) {
+        orc_mat3_mul(t, x, t);
+        for (int k = 0; k < 9; ++k) {
+            t[k] = t[k] / (double)j;
+            e[k] = e[k] + t[k];
+        }
+    }
+    for (int q = 0; q < sq; ++q) orc_mat3_mul(e, e, e);
+    memcpy(out, e, sizeof(e));
+}
+
+void orc_mtk_scale(const orc_mtk* m, double dt, int32_t velocity_scaling, double* out)
+{
+    double eta[9], a[9];
+    for (int k = 0; k < 9; ++k) eta[k] = m->momentum[k] / m->w;
+    symmetrize(eta, eta);
+    const double factor = velocity_scaling ? -0.5 : 1.0;
+    for (int k = 0; k < 9; ++k) a[k] = eta[k] * factor * dt;
+    orc_mat3_exp(a, out);
+}
+
+/* (M M^T).trace() / (2 w) */
+double orc_mtk_kinetic_energy(const orc_mtk* m)
+{
+    double mt[9], p[9];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) M(mt, r, c) = M(m->momentum, c, r);
+    orc_mat3_mul(m->momentum, mt, p);
+    return ((M(p, 0, 0) + M(p, 1, 1)) + M(p, 2, 2)) / (2.0 * m->w);
+}
+
+/* (P0^T h).trace() */
+double orc_mtk_potential_energy(const orc_mtk* m, const double* h)
+{
+    double tt[9], p[9];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) M(tt, r, c) = M(m->target, c, r);
+    orc_mat3_mul(tt, h, p);
+    return (M(p, 0, 0) + M(p, 1, 1)) + M(p, 2, 2);
+}
+
+/* s = h^-1 r per atom; h = scale h; h^-1 = inverse(h); r = h s. */
+void orc_scale_box(orc_sys* s, const double* scale)
+{
+    double* sf = (double*)malloc(sizeof(double) * 3 * (size_t)s->n);
+    if (!sf) return;
+    for (int64_t i = 0; i < s->n; ++i) orc_gemv(s->box.hinv, &s->pos[3 * i], &sf[3 * i]);
+    orc_mat3_mul(scale, s->box.h, s->box.h);
+    orc_box_inverse(s->box.h, s->box.hinv);
+    for (int64_t i = 0; i < s->n; ++i) orc_gemv(s->box.h, &sf[3 * i], &s->pos[3 * i]);
+    free(sf);
+}
+
+static void scale_velocities(orc_sys* s, const double* scale)
+{
+    for (int64_t i = 0; i < s->n; ++i) {
+        double v[3];
+        orc_gemv(scale, &s->vel[3 * i], v);
+        memcpy(&s->vel[3 * i], v, sizeof(v));
+    }
+}
+
+double orc_verlet_step_npt_mtk(orc_sys* s, double dt, orc_mtk* m, orc_nhc* c, int32_t manager)
+{
+    double d[9], scale[9], scale_h[9];
+    orc_mtk_delta_momentum(m, s, dt, d);
+    for (int k = 0; k < 9; ++k) m->momentum[k] += d[k];
+    orc_mtk_scale(m, dt, 1, scale);
+    scale_velocities(s, scale);
+    orc_mtk_scale(m, dt, 0, scale_h);
+    orc_scale_box(s, scale_h);
+    const double pe = orc_verlet_step_nvt_nhc(s, dt, c, manager);
+    scale_velocities(s, scale);
+    orc_mtk_delta_momentum(m, s, dt, d);
+    for (int k = 0; k < 9; ++k) m->momentum[k] += d[k];
+    return pe;
+}
+
+int orc_run_npt(orc_sys* s, double dt, int64_t nsteps, int32_t manager, double kb, double t_start,
+                double t_end, double tau_t, int32_t chain, double p0, double tau_p, double* thermo,
+                double* hbox)
+{
+    orc_nhc c;
+    orc_mtk m;
+    if (orc_nhc_new(&c, t_start, t_end, tau_t, chain, kb) != 0) return -2;
+    orc_mtk_new(&m, p0, tau_p, s->n, t_start);
+    for (int64_t k = 0; k < 3 * s->n; ++k) s->force[k] = 0.0;
+    double pe0 = orc_compute(s, manager);
+    if (pe0 != pe0) return -1;
+    for (int64_t t = 0; t < nsteps; ++t) {
+        const double pe = orc_verlet_step_npt_mtk(s, dt, &m, &c, manager);
+        orc_nhc_target(&c, t, nsteps);
+        const double ke = orc_kinetic_energy(s);
+        if (thermo) {
+            thermo[5 * t + 0] = pe;
+            thermo[5 * t + 1] = ke;
+            thermo[5 * t + 2] = pe + ke + orc_nhc_kinetic_energy(&c) + orc_nhc_potential_energy(&c, s->n) +
+                                orc_mtk_kinetic_energy(&m) + orc_mtk_potential_energy(&m, s->box.h);
+            thermo[5 * t + 3] = orc_temperature(s, ke, kb);
+            thermo[5 * t + 4] = orc_pressure(s, ke);
+        }
+        if (hbox) memcpy(&hbox[9 * t], s->box.h, sizeof(double) * 9);
+    }
+    return 0;
+}

@@ -135,6 +135,37 @@ double orc_verlet_step_nvt_nhc(orc_sys* s, double dt, orc_nhc* c, int32_t manage
 int orc_run_nvt(orc_sys* s, double dt, int64_t nsteps, int32_t manager, double kb, double t_start,
                 double t_end, double tau, int32_t chain, double* thermo);
 
+/* MTK barostat (ensemble/npt.rs:10-88) and the NPT step (potential.rs:112-135).
+ * Tensors are 3x3 column-major.  The matrix exponential is this oracle's own method
+ * (scaling and squaring of a Taylor series; nalgebra 0.34.1's Pade-based exp() is not
+ * vendored in the reference): parity of exp() and of the tensor sums is tolerance-based. */
+typedef struct {
+    double target[9];   /* target pressure tensor: p0 * I for `fix ... iso p0 p1 tau` */
+    double momentum[9]; /* barostat momentum */
+    double w;           /* barostat mass */
+} orc_mtk;
+
+/* MTKBarostat::new via new_from_args (npt.rs:24-42, 66-87): w = 3n KB T tau^2 with T the
+ * chain's start temperature (300 without a chain) and KB = 0.0083144621. */
+void orc_mtk_new(orc_mtk* m, double p0, double tau, int64_t n, double t_target);
+void orc_kinetic_tensor(const orc_sys* s, double* out);   /* properties.rs:41-43, no masses */
+void orc_virial_tensor(const orc_sys* s, double* out);    /* properties.rs:45-47 */
+void orc_pressure_tensor(const orc_sys* s, double* out);  /* properties.rs:53-59 */
+void orc_mtk_delta_momentum(const orc_mtk* m, const orc_sys* s, double dt, double* out); /* npt.rs:44-49 */
+void orc_mtk_scale(const orc_mtk* m, double dt, int32_t velocity_scaling, double* out);  /* npt.rs:51-58 */
+double orc_mtk_kinetic_energy(const orc_mtk* m);                 /* npt.rs:60-62 */
+double orc_mtk_potential_energy(const orc_mtk* m, const double* h); /* npt.rs:64-66 */
+void orc_mat3_mul(const double* a, const double* b, double* out);   /* nalgebra gemm order */
+void orc_mat3_exp(const double* a, double* out);
+void orc_scale_box(orc_sys* s, const double* scale);            /* transformations.rs:6-15 */
+double orc_verlet_step_npt_mtk(orc_sys* s, double dt, orc_mtk* m, orc_nhc* c, int32_t manager);
+/* Simulation::run with Ensemble::NPT (simulation.rs:8-64, 90-116): per step writes PE, KE,
+ * H = PE + KE + chain KE + chain PE + barostat KE + barostat PE, T, P into thermo[5*k] and,
+ * when hbox is given, the box matrix after the step into hbox[9*k]. */
+int orc_run_npt(orc_sys* s, double dt, int64_t nsteps, int32_t manager, double kb, double t_start,
+                double t_end, double tau_t, int32_t chain, double p0, double tau_p, double* thermo,
+                double* hbox);
+
 /* Idealised parallel CPU baseline (not a restatement): full list + OpenMP gather. */
 int orc_ideal_run_nve(orc_sys* s, double dt, int64_t nsteps, double skin, int32_t nthreads,
                       double* pe_out);

@@ -60,6 +60,11 @@ class NHC(C.Structure):
     ]
 
 
+class MTK(C.Structure):
+    """orc_mtk: MTK barostat state (ensemble/npt.rs:10-20)."""
+    _fields_ = [("target", C.c_double * 9), ("momentum", C.c_double * 9), ("w", C.c_double)]
+
+
 class List(C.Structure):
     _fields_ = [
         ("n", C.c_int64),
@@ -112,6 +117,21 @@ def lib():
             "orc_nhc_potential_energy": (C.c_double, [C.POINTER(NHC), C.c_int64]),
             "orc_nhc_target": (None, [C.POINTER(NHC), C.c_int64, C.c_int64]),
             "orc_verlet_step_nvt_nhc": (C.c_double, [C.POINTER(Sys), C.c_double, C.POINTER(NHC), C.c_int32]),
+            "orc_mtk_new": (None, [C.POINTER(MTK), C.c_double, C.c_double, C.c_int64, C.c_double]),
+            "orc_kinetic_tensor": (None, [C.POINTER(Sys), _D]),
+            "orc_virial_tensor": (None, [C.POINTER(Sys), _D]),
+            "orc_pressure_tensor": (None, [C.POINTER(Sys), _D]),
+            "orc_mtk_delta_momentum": (None, [C.POINTER(MTK), C.POINTER(Sys), C.c_double, _D]),
+            "orc_mtk_scale": (None, [C.POINTER(MTK), C.c_double, C.c_int32, _D]),
+            "orc_mtk_kinetic_energy": (C.c_double, [C.POINTER(MTK)]),
+            "orc_mtk_potential_energy": (C.c_double, [C.POINTER(MTK), _D]),
+            "orc_mat3_mul": (None, [_D, _D, _D]),
+            "orc_mat3_exp": (None, [_D, _D]),
+            "orc_scale_box": (None, [C.POINTER(Sys), _D]),
+            "orc_verlet_step_npt_mtk": (C.c_double, [C.POINTER(Sys), C.c_double, C.POINTER(MTK), C.POINTER(NHC),
+                                                     C.c_int32]),
+            "orc_run_npt": (C.c_int, [C.POINTER(Sys), C.c_double, C.c_int64, C.c_int32, C.c_double, C.c_double,
+                                      C.c_double, C.c_double, C.c_int32, C.c_double, C.c_double, _D, _D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -253,6 +273,32 @@ class OSystem:
             raise RuntimeError("oracle NVT run failed")
         return th
 
+    def run_npt(self, dt, nsteps, t_start, t_end, tau_t, p0, tau_p, chain=3, manager=MGR_OFFSET, kb=1.0):
+        """Simulation::run with Ensemble::NPT; returns thermo [nsteps, 5] and the box matrix
+        after every step [nsteps, 9] (column-major)."""
+        th = np.zeros((nsteps, 5))
+        hb = np.zeros((nsteps, 9))
+        rc = lib().orc_run_npt(C.byref(self.s), dt, nsteps, manager, kb, t_start, t_end, tau_t, chain, p0, tau_p,
+                               th.ctypes.data_as(_D), hb.ctypes.data_as(_D))
+        if rc != 0:
+            raise RuntimeError("oracle NPT run failed")
+        return th, hb
+
+    def tensor(self, which):
+        """'kinetic', 'virial' or 'pressure' tensor of the current state (3x3, row-major view)."""
+        out = np.zeros(9)
+        f = {"kinetic": lib().orc_kinetic_tensor, "virial": lib().orc_virial_tensor,
+             "pressure": lib().orc_pressure_tensor}[which]
+        f(C.byref(self.s), out.ctypes.data_as(_D))
+        return out.reshape(3, 3).T.copy()
+
+    def box_h(self):
+        return np.array(self.s.box.h[:]).reshape(3, 3).T.copy()
+
+    def scale_box(self, scale):
+        m = np.ascontiguousarray(np.asarray(scale, dtype=np.float64).T.reshape(9))
+        lib().orc_scale_box(C.byref(self.s), m.ctypes.data_as(_D))
+
     def ideal_run_nve(self, dt, nsteps, skin, nthreads=0):
         pe = C.c_double(0.0)
         if lib().orc_ideal_run_nve(C.byref(self.s), dt, nsteps, skin, nthreads, C.byref(pe)) != 0:
@@ -310,3 +356,11 @@ def csr_to_sorted_pairs(start, idx) -> np.ndarray:
     pairs = np.stack([i, np.asarray(idx, dtype=np.int64)], axis=1)
     order = np.lexsort((pairs[:, 1], pairs[:, 0]))
     return pairs[order]
+
+
+def mat3_exp(a):
+    """The oracle's 3x3 matrix exponential (row-major in, row-major out)."""
+    m = np.ascontiguousarray(np.asarray(a, dtype=np.float64).T.reshape(9))
+    out = np.zeros(9)
+    lib().orc_mat3_exp(m.ctypes.data_as(_D), out.ctypes.data_as(_D))
+    return out.reshape(3, 3).T.copy()

@@ -516,3 +516,81 @@ def test_nvt_with_a_stiff_chain_reduces_to_nve():
     th_nve = b.run_nve(0.005, 50, po.MGR_OFFSET, kb=1.0)
     assert np.abs(th_nvt[:, 0] - th_nve[:, 0]).max() <= 1e-9 * np.abs(th_nve[:, 0]).max()
     assert np.abs(a.pos - b.pos).max() <= 1e-12
+
+
+# ---- MTK barostat / NPT (ensemble/npt.rs, potential.rs:112-135) ---------------------
+
+
+@pytest.mark.parametrize("scale", [1e-6, 0.3, 4.0])
+def test_matrix_exponential_matches_scipy(scale):
+    import scipy.linalg as sl
+
+    rng = np.random.default_rng(7)
+    for _ in range(20):
+        a = rng.normal(size=(3, 3)) * scale
+        want = sl.expm(a)
+        # a large norm (scaling and squaring) loses digits to the problem's conditioning
+        tol = 1e-14 if scale < 1.0 else 1e-12
+        assert np.abs(po.mat3_exp(a) - want).max() <= tol * max(1.0, np.abs(want).max())
+    assert np.array_equal(po.mat3_exp(np.zeros((3, 3))), np.eye(3))
+    d = np.diag([0.1, -0.2, 0.3])
+    assert np.allclose(po.mat3_exp(d), np.diag(np.exp([0.1, -0.2, 0.3])), rtol=1e-15, atol=0)
+
+
+def test_pressure_tensors_against_numpy_sums():
+    import ctypes as C
+
+    spec = W.lj_box(200, 0.8, 1.0, seed=41)
+    spec.masses = np.ones_like(spec.masses)
+    o = po.OSystem.from_spec(spec)
+    o.compute(po.MGR_OFFSET)
+    kt, wt, pt = o.tensor("kinetic"), o.tensor("virial"), o.tensor("pressure")
+    assert np.allclose(kt, o.vel.T @ o.vel, rtol=1e-12, atol=1e-12)     # no masses (quirk)
+    assert np.allclose(wt, o.pos.T @ o.force, rtol=1e-12, atol=1e-10)
+    vol = abs(np.linalg.det(o.box_h()))
+    assert np.allclose(pt, (kt + wt) / vol, rtol=1e-12, atol=1e-14)
+    # unit masses: tr(P) = 3 P with P = (2 KE + tr W) / (3 V)
+    ke = o.kinetic_energy()
+    assert np.trace(pt) == pytest.approx(3.0 * po.lib().orc_pressure(C.byref(o.s), ke), rel=1e-12)
+
+
+def test_scale_box_keeps_fractional_coordinates():
+    spec = W.lj_box(300, 0.8, 1.0, seed=42)
+    o = po.OSystem.from_spec(spec)
+    h0 = o.box_h()
+    s0 = o.pos @ np.linalg.inv(h0).T
+    scale = po.mat3_exp(np.array([[1e-3, 2e-4, 0.0], [2e-4, -5e-4, 1e-4], [0.0, 1e-4, 2e-3]]))
+    o.scale_box(scale)
+    h1 = o.box_h()
+    assert np.allclose(h1, scale @ h0, rtol=1e-15, atol=1e-14)
+    assert np.allclose(o.pos @ np.linalg.inv(h1).T, s0, rtol=0, atol=1e-13)
+    assert np.allclose(np.array(o.s.box.hinv[:]).reshape(3, 3).T @ h1, np.eye(3), atol=1e-15)
+
+
+def test_npt_with_a_heavy_barostat_reduces_to_nvt():
+    # tau_p -> infinity: w huge, the barostat scalings are the identity to rounding
+    spec = W.lj_liquid(5, 0.8442, 0.72, seed=43)
+    a = po.OSystem.from_spec(spec)
+    b = po.OSystem.from_spec(spec)
+    th_npt, hb = a.run_npt(0.004, 40, 0.72, 0.72, 0.5, 0.0, 1e12, 3, kb=1.0)
+    th_nvt = b.run_nvt(0.004, 40, 0.72, 0.72, 0.5, 3, kb=1.0)
+    assert np.abs(th_npt[:, :2] - th_nvt[:, :2]).max() <= 1e-10 * np.abs(th_nvt[:, 0]).max()
+    assert np.abs(hb[-1] - np.asarray(spec.box.h).ravel(order="F")).max() <= 1e-12
+
+
+@pytest.mark.parametrize("p_target, grows", [(0.0, False), (6.0, True)])
+def test_npt_volume_follows_the_pressure_difference(p_target, grows):
+    # the instantaneous pressure of this liquid is ~1.4: a lower target lets the box expand
+    # (P - P0 > 0 pushes the momentum up), a higher one compresses it.  The barostat mass
+    # uses the constant KB = 0.0083 (npt.rs:32), so in these units it is light.
+    spec = W.lj_liquid(5, 0.8442, 0.9, seed=44)
+    o = po.OSystem.from_spec(spec)
+    v0 = abs(np.linalg.det(o.box_h()))
+    th, hb = o.run_npt(0.004, 100, 0.9, 0.9, 0.5, p_target, 10.0, 3, kb=1.0)
+    v1 = abs(np.linalg.det(hb[-1].reshape(3, 3).T))
+    assert (v1 > v0) == (not grows)
+    assert np.isfinite(th).all()
+    # the box picks up tilt from the symmetrised off-diagonal pressure (about 1 % of the
+    # edge for 500 atoms in 100 steps): NPT needs a general box matrix
+    h = hb[-1].reshape(3, 3).T
+    assert 0.0 < np.abs(h - np.diag(np.diag(h))).max() < 5e-2 * np.abs(np.diag(h)).min()
