@@ -221,6 +221,30 @@ pis_status pis_batch_verlet_step_nvt_ramp(pis_ctx* ctx, const double* dts, uint6
  * host copy of the chain (either pointer may be NULL). */
 pis_status pis_nvt_get_chain(pis_ctx* ctx, uint32_t sys, double* xi, double* target);
 
+/* ---- NPT: MTK barostat (ensemble/npt.rs:10-88) around the NVT step ---------------------
+ * `fix ... npt temp T0 T1 tau_t iso p0 p1 tau_p`: target pressure p0 * I (commands.rs:429-446;
+ * p1 is parsed and unused, like the reference's end_pressure). */
+typedef struct pis_mtk_params {
+    double p_start;     /* target pressure p0 */
+    double tau;         /* barostat time constant; w = 3N * 0.0083144621 * T0 * tau^2 */
+    double reserved[2]; /* must be 0 */
+} pis_mtk_params;
+/* MTKBarostat::new_from_args + NHThermostatChain::new_from_args for every system
+ * (replaces Simulation::run's construction, simulation.rs:17-22). */
+pis_status pis_npt_init(pis_ctx* ctx, const pis_nhc_params* chain, const pis_mtk_params* barostat);
+/* nsteps of verlet_step_npt_mtk (potential.rs:112-135) with the chain's target ramp as in
+ * pis_batch_verlet_step_nvt_ramp (ramp_total 0: this call is the whole run).  The box is
+ * general after the first step (the symmetrised pressure tensor has off-diagonal terms);
+ * the list is rebuilt every step for the new box, as the reference's force call does.
+ * The record's etot is the NPT Hamiltonian (simulation.rs:106-114).  PIS_ERR_CAPACITY if
+ * the box gets thinner than the cell grid planned for it. */
+pis_status pis_batch_verlet_step_npt(pis_ctx* ctx, const double* dts, uint64_t nsteps,
+                                     uint64_t ramp_step0, uint64_t ramp_total,
+                                     pis_thermo* thermo_out);
+/* Current box matrix h (column-major, SimulationBox::h) and barostat momentum of system
+ * `sys` (either pointer may be NULL). */
+pis_status pis_npt_get_box(pis_ctx* ctx, uint32_t sys, double* h_out, double* momentum_out);
+
 /* ---- Serving: continuous batching of NVE jobs on one GPU (no reference counterpart; the
  * reference runs one Simulation::run per process, simulation.rs:33-105).  A worker thread
  * keeps the admitted jobs' state resident and advances all of them together in rounds

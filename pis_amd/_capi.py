@@ -102,6 +102,10 @@ class Stats(C.Structure):
     ]
 
 
+class MTKParams(C.Structure):
+    _fields_ = [("p_start", C.c_double), ("tau", C.c_double), ("reserved", C.c_double * 2)]
+
+
 class NHCParams(C.Structure):
     _fields_ = [
         ("t_start", C.c_double),
@@ -178,6 +182,10 @@ SIGNATURES = {
     "pis_nvt_get_chain": (C.c_int, [_P, C.c_uint32, _DP, _DP]),
     "pis_batch_verlet_step_nvt_ramp": (
         C.c_int, [_P, _DP, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(Thermo)]),
+    "pis_npt_init": (C.c_int, [_P, C.POINTER(NHCParams), C.POINTER(MTKParams)]),
+    "pis_batch_verlet_step_npt": (
+        C.c_int, [_P, _DP, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(Thermo)]),
+    "pis_npt_get_box": (C.c_int, [_P, C.c_uint32, _DP, _DP]),
     "pis_server_create": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(_P)]),
     "pis_server_submit": (
         C.c_int,

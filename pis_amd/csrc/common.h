@@ -38,10 +38,31 @@ enum {
     CTL_REMAIN = 10,   // steps left in the current call (the force kernel advances the
                        // positions of the next step while more steps follow)
     CTL_REBUILD_NEXT = 11,  // skin check of those advanced positions
-    CTL_COUNT = 12
+    CTL_BOX = 12,      // the box (NPT) became too thin for the planned cell grid
+    CTL_COUNT = 13
 };
 
+// Device-resident box of a system (SysDev::box): h and h^-1 column-major, |det h|.  The
+// host's bits on upload; the barostat updates them on the device.
+enum { BOX_H = 0, BOX_HINV = 9, BOX_VOL = 18, BOX_COUNT = 20 };
+
+// MTK barostat state per system (SysDev::npt, ensemble/npt.rs:10-20), device-updated.
+enum {
+    NPT_TARGET = 0,     // target pressure tensor (column-major)
+    NPT_MOM = 9,        // barostat momentum
+    NPT_W = 18,         // barostat mass
+    NPT_SCALE = 19,     // velocity scaling matrix of the current step
+    NPT_HINV_OLD = 28,  // h^-1 before the current step's box scaling
+    NPT_RCL = 37,       // cell edge the grid needs (rcut + skin with margin)
+    NPT_COUNT = 40
+};
+// per-block partial sums of the barostat kernels (SysDev::tens): v v^T, r F^T, (1/2) m v^2
+enum { TENS_K = 0, TENS_W = 9, TENS_KE = 18, TENS_COUNT = 20 };
+
 enum { MODE_TILE_REL = 0, MODE_RAW_MI = 1, MODE_WHOLE = 2 };
+// kinds of systems a launch covers: the build and force kernels come in an orthorhombic and
+// a general-box instantiation, each taking only its own systems' tiles
+enum { KIND_ORTHO = 1, KIND_TRI = 2 };
 
 struct PairParam {
     double r2_force;  // largest r^2 with sqrt_rn(r^2) <= rcut (-1: pair absent)
@@ -121,6 +142,12 @@ struct SysDev {
     int32_t thermo_cap;
     int32_t nhc_on;        // NVT: apply the chain's velocity scaling in the step kernels
     double* nhc;           // [NHC_COUNT] Nose-Hoover chain state (device-updated)
+    int32_t tri;           // general (tilted) box: minimum image, wrap and binning use box[]
+    int32_t npt_on;        // NPT: the chain's start-of-step KE comes from the barostat's sums
+    double* box;           // [BOX_COUNT] h, h^-1, volume (device-updated under NPT)
+    double* npt;           // [NPT_COUNT] barostat state
+    double* tens;          // [nblk * TENS_COUNT] per-atom-block partial sums
+    int32_t nblk;          // 256-atom blocks (npad / 256, rounded up)
 };
 
 // Work items of batched launches.

@@ -37,6 +37,11 @@ struct Plan {
     double L[3] = {0, 0, 0}, Linv[3] = {0, 0, 0}, cellw[3] = {0, 0, 0};
     double rcmax = 0, rcl = 0, skin = 0, density = 0, volume = 0, kb = 1, wslack = 0;
     bool mixed = false;
+    // general box (off-diagonal h): h, h^-1 as given (column-major); L holds the
+    // perpendicular widths, and qmax bounds |x| of a wrapped position per component
+    bool tri = false;
+    double h[9] = {0}, hinv[9] = {0};
+    double qmax = 0;
     std::vector<PairParam> pp;
     std::vector<double> mass;
     std::vector<uint8_t> type0;
@@ -88,6 +93,7 @@ struct pis_ctx {
     // update kernel) and its later NVE steps (positions already advanced by the force kernel)
     hipGraphExec_t graph[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     bool nhc_ready = false;  // pis_nvt_init has set every system's chain
+    bool npt_ready = false;  // pis_npt_init has set every system's barostat
     // staging classes after the first run their list build and force on streams of their
     // own (default priority), forked from and joined back into `stream` by events
     std::vector<hipStream_t> side;
@@ -118,6 +124,13 @@ pis_status ensure_xfer(pis_ctx* c, size_t bytes);
 pis_status read_ctl(pis_ctx* c, uint32_t k, int32_t* ctl);
 pis_status write_ctl(pis_ctx* c, uint32_t k, const int32_t* ctl);
 pis_status enqueue_classes(pis_ctx* c, bool build, int force);
+// KIND_* bits of the context's systems (which build/force instantiations a launch needs)
+inline int ctx_kinds(const pis_ctx* c)
+{
+    int k = 0;
+    for (const auto& h : c->sys) k |= h.plan.tri ? KIND_TRI : KIND_ORTHO;
+    return k;
+}
 pis_status enqueue_rebuild(pis_ctx* c);
 pis_status run_rebuild(pis_ctx* c);
 // Status of every system (or only system `only`) from its device flags.

@@ -100,6 +100,64 @@ void choose_tiles(Plan& p)
     for (int d = 0; d < 3; ++d) p.tile[d] = bt[d];
 }
 
+// Rounding bands of every pair threshold.  FP64: r^2 computed from staged (tile-relative
+// or raw) coordinates versus the reference's r^2 -- the displacement differs by at most
+// ~4 ulp(qmax) in each component plus a few relative ulps, so relative to r^2 at distance
+// r the gap is below 8 qmax 2^-52 / r + 11 * 2^-53; the band is 16 times that (twice for a
+// general box, whose minimum image takes two 3-term products).
+void set_bands(Plan* p)
+{
+    {
+        const double Lmax = p->qmax * (p->tri ? 2.0 : 1.0);
+        const double u = std::ldexp(1.0, -52);
+        auto band = [&](double r) { return 16.0 * (8.0 * Lmax * u / r + 5.5 * u); };
+        for (auto& pp : p->pp) {
+            if (pp.r2_force < 0.0) {
+                pp.f_lo = pp.f_hi = pp.l_lo = pp.l_hi = -1.0;
+                continue;
+            }
+            const double bf = band(std::sqrt(pp.r2_force)), bl = band(std::sqrt(pp.r2_list));
+            pp.f_lo = pp.r2_force * (1.0 - bf);
+            pp.f_hi = pp.r2_force * (1.0 + bf);
+            pp.l_lo = pp.r2_list * (1.0 - bl);
+            pp.l_hi = pp.r2_list * (1.0 + bl);
+        }
+    }
+    // Band for r^2 from the list build's FP32 staging.  Staged coordinates lie within
+    // qmax of the origin (the halo half-extent for tile-relative staging, the box for raw
+    // positions) and round to FP32 with error 2^-24 qmax, so a displacement component is
+    // within e = 6 * 2^-24 * qmax of its exact value (difference and image shift rounded
+    // once more), and at distance r, r^2 is within 2 sqrt(3) e r + 3 e^2 + 3 * 2^-24 r^2.
+    // The band is 4 times that; the x windows widen by twice the coordinate error.
+    {
+        const double u32 = std::ldexp(1.0, -24);
+        double qmax = 0.0, cwmax = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            qmax = std::max(qmax, p->mode == MODE_TILE_REL ? 0.5 * (p->tile[k] + 2) * p->cellw[k] : p->L[k]);
+            cwmax = std::max(cwmax, p->cellw[k]);
+        }
+        // a general box: positions up to qmax per component, and the FP32 image takes two
+        // 3-term products (twice the error)
+        if (p->tri) qmax = p->qmax;
+        const double e = (p->tri ? 12.0 : 6.0) * u32 * qmax;
+        p->wslack = 2.0 * u32 * qmax + 1e-9 * cwmax;
+        for (auto& q : p->pp) {
+            if (q.r2_list < 0.0) {
+                q.l_lo32 = q.l_hi32 = -1.0f;
+                continue;
+            }
+            const double R = std::sqrt(q.r2_list);
+            const double beta = 4.0 * ((2.0 * std::sqrt(3.0) * e * R + 3.0 * e * e) / q.r2_list + 3.0 * u32);
+            const double lo = q.r2_list * (1.0 - beta), hi = q.r2_list * (1.0 + beta);
+            float flo = (float)lo, fhi = (float)hi;
+            if ((double)flo > lo) flo = std::nextafter(flo, -INFINITY);
+            if ((double)fhi < hi) fhi = std::nextafter(fhi, INFINITY);
+            q.l_lo32 = std::max(flo, 0.0f);
+            q.l_hi32 = fhi;
+        }
+    }
+}
+
 pis_status make_plan_impl(const pis_system_desc* d, Plan* p)
 {
     if (!d) return fail(PIS_ERR_INVALID_ARGUMENT, "null system descriptor");
@@ -115,10 +173,16 @@ pis_status make_plan_impl(const pis_system_desc* d, Plan* p)
     if (d->precision > 1) return fail(PIS_ERR_INVALID_ARGUMENT, "unknown precision");
     if (!d->pbc[0] || !d->pbc[1] || !d->pbc[2])
         return fail(PIS_ERR_UNSUPPORTED, "only fully periodic boxes are supported");
+    p->tri = false;
     for (int r = 0; r < 3; ++r)
         for (int c = 0; c < 3; ++c)
-            if (r != c && (d->h[c * 3 + r] != 0.0 || d->h_inv[c * 3 + r] != 0.0))
-                return fail(PIS_ERR_UNSUPPORTED, "triclinic boxes are not supported");
+            if (r != c && (d->h[c * 3 + r] != 0.0 || d->h_inv[c * 3 + r] != 0.0)) p->tri = true;
+    for (int k = 0; k < 9; ++k) {
+        p->h[k] = d->h[k];
+        p->hinv[k] = d->h_inv[k];
+        if (!std::isfinite(d->h[k]) || !std::isfinite(d->h_inv[k]))
+            return fail(PIS_ERR_INVALID_ARGUMENT, "box matrix must be finite");
+    }
     if (!(d->skin >= 0.0) || !std::isfinite(d->skin)) return fail(PIS_ERR_INVALID_ARGUMENT, "bad skin");
     if (!(d->kb > 0.0)) return fail(PIS_ERR_INVALID_ARGUMENT, "kb must be > 0");
     const int nt = (int)d->n_types;
@@ -127,13 +191,35 @@ pis_status make_plan_impl(const pis_system_desc* d, Plan* p)
     p->skin = d->skin;
     p->kb = d->kb;
     p->mixed = d->precision == PIS_PRECISION_MIXED;
-    for (int k = 0; k < 3; ++k) {
-        p->L[k] = d->h[k * 3 + k];
-        p->Linv[k] = d->h_inv[k * 3 + k];
-        if (!(p->L[k] > 0.0) || !(p->Linv[k] > 0.0) || !std::isfinite(p->L[k]))
-            return fail(PIS_ERR_INVALID_ARGUMENT, "box lengths must be positive");
+    if (p->tri) {
+        // SimulationBox::volume (simulation_box.rs:67-69): |det h|; cells follow the
+        // perpendicular widths V / |a_i x a_j| of the box
+        const double* h = d->h;
+        const double m11 = h[4] * h[8] - h[5] * h[7];
+        const double m12 = h[1] * h[8] - h[2] * h[7];
+        const double m13 = h[1] * h[5] - h[2] * h[4];
+        p->volume = std::fabs(h[0] * m11 - h[3] * m12 + h[6] * m13);
+        if (!(p->volume > 0.0)) return fail(PIS_ERR_INVALID_ARGUMENT, "box matrix is singular");
+        for (int k = 0; k < 3; ++k) {
+            const double* a = h + 3 * ((k + 1) % 3);
+            const double* b = h + 3 * ((k + 2) % 3);
+            const double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
+            p->L[k] = p->volume / std::sqrt(cx * cx + cy * cy + cz * cz);
+            p->Linv[k] = 1.0 / p->L[k];
+        }
+        p->qmax = 0.0;
+        for (int r = 0; r < 3; ++r)
+            p->qmax = std::max(p->qmax, std::fabs(h[r]) + std::fabs(h[3 + r]) + std::fabs(h[6 + r]));
+    } else {
+        for (int k = 0; k < 3; ++k) {
+            p->L[k] = d->h[k * 3 + k];
+            p->Linv[k] = d->h_inv[k * 3 + k];
+            if (!(p->L[k] > 0.0) || !(p->Linv[k] > 0.0) || !std::isfinite(p->L[k]))
+                return fail(PIS_ERR_INVALID_ARGUMENT, "box lengths must be positive");
+        }
+        p->volume = std::fabs(p->L[0] * p->L[1] * p->L[2]);
+        p->qmax = std::max(p->L[0], std::max(p->L[1], p->L[2]));
     }
-    p->volume = std::fabs(p->L[0] * p->L[1] * p->L[2]);
     p->mass.resize(nt);
     for (int t = 0; t < nt; ++t) {
         if (!(d->masses[t] > 0.0)) return fail(PIS_ERR_INVALID_ARGUMENT, "masses must be > 0");
@@ -187,31 +273,6 @@ pis_status make_plan_impl(const pis_system_desc* d, Plan* p)
     }
     p->rcl = p->rcmax + d->skin;
     p->density = (double)p->n / p->volume;
-    // Rounding band of r^2 computed from staged (tile-relative or raw) coordinates versus
-    // the reference's r^2: the displacement differs by at most ~4 ulp(L) in each component
-    // plus a few relative ulps, so relative to r^2 at distance r the gap is below
-    // 8 L 2^-52 / r + 11 * 2^-53.  The band is 16 times that.
-    {
-        const double Lmax = std::max(p->L[0], std::max(p->L[1], p->L[2]));
-        const double u = std::ldexp(1.0, -52);
-        auto band = [&](double r) { return 16.0 * (8.0 * Lmax * u / r + 5.5 * u); };
-        for (uint32_t k = 0; k < d->n_pairs; ++k) {
-            const pis_pair_coeff& c = d->pairs[k];
-            const int a = (int)std::min(c.type_i, c.type_j) - 1, b = (int)std::max(c.type_i, c.type_j) - 1;
-            for (int q : {a * nt + b, b * nt + a}) {
-                PairParam& pp = p->pp[(size_t)q];
-                const double bf = band(c.rcut), bl = band(c.rcut + d->skin);
-                pp.f_lo = pp.r2_force * (1.0 - bf);
-                pp.f_hi = pp.r2_force * (1.0 + bf);
-                pp.l_lo = pp.r2_list * (1.0 - bl);
-                pp.l_hi = pp.r2_list * (1.0 + bl);
-            }
-        }
-        for (auto& q : p->pp)
-            if (q.r2_force < 0.0) {
-                q.f_lo = q.f_hi = q.l_lo = q.l_hi = -1.0;
-            }
-    }
     // Cell grid: edge strictly larger than rcut + skin (relative margin far above
     // rounding) so every pair within the list cutoff is in adjacent cells.
     bool thin = false;
@@ -243,41 +304,13 @@ pis_status make_plan_impl(const pis_system_desc* d, Plan* p)
             if (!(p->tile[k] * p->cellw[k] + p->skin + 2.0 * p->rcmax < p->L[k] * (1.0 - 1e-9)))
                 p->mode = MODE_RAW_MI;
         }
+        if (p->tri) p->mode = MODE_RAW_MI;  // raw positions, per-pair image of the general box
         const double stage = p->density * cellvol * (p->tile[0] + 2) * (p->tile[1] + 2) * (p->tile[2] + 2);
         p->scap = scap_for(stage * 1.1 + 48);
         if (p->scap < 0) return fail(PIS_ERR_UNSUPPORTED, "tile staging exceeds LDS capacity");
     }
     p->ntile = p->nt[0] * p->nt[1] * p->nt[2];
-    // Band for r^2 from the list build's FP32 staging.  Staged coordinates lie within
-    // qmax of the origin (the halo half-extent for tile-relative staging, the box for raw
-    // positions) and round to FP32 with error 2^-24 qmax, so a displacement component is
-    // within e = 6 * 2^-24 * qmax of its exact value (difference and image shift rounded
-    // once more), and at distance r, r^2 is within 2 sqrt(3) e r + 3 e^2 + 3 * 2^-24 r^2.
-    // The band is 4 times that; the x windows widen by twice the coordinate error.
-    {
-        const double u32 = std::ldexp(1.0, -24);
-        double qmax = 0.0, cwmax = 0.0;
-        for (int k = 0; k < 3; ++k) {
-            qmax = std::max(qmax, p->mode == MODE_TILE_REL ? 0.5 * (p->tile[k] + 2) * p->cellw[k] : p->L[k]);
-            cwmax = std::max(cwmax, p->cellw[k]);
-        }
-        const double e = 6.0 * u32 * qmax;
-        p->wslack = 2.0 * u32 * qmax + 1e-9 * cwmax;
-        for (auto& q : p->pp) {
-            if (q.r2_list < 0.0) {
-                q.l_lo32 = q.l_hi32 = -1.0f;
-                continue;
-            }
-            const double R = std::sqrt(q.r2_list);
-            const double beta = 4.0 * ((2.0 * std::sqrt(3.0) * e * R + 3.0 * e * e) / q.r2_list + 3.0 * u32);
-            const double lo = q.r2_list * (1.0 - beta), hi = q.r2_list * (1.0 + beta);
-            float flo = (float)lo, fhi = (float)hi;
-            if ((double)flo > lo) flo = std::nextafter(flo, -INFINITY);
-            if ((double)fhi < hi) fhi = std::nextafter(fhi, INFINITY);
-            q.l_lo32 = std::max(flo, 0.0f);
-            q.l_hi32 = fhi;
-        }
-    }
+    set_bands(p);
     p->npad = (int)((p->n + 63) / 64 * 64);
     int maxnb;
     if (d->neighbor_capacity)
@@ -309,7 +342,7 @@ namespace {
 // Carving of one system's allocation (offsets of every array, 256-byte aligned).
 struct Layout {
     size_t st[2][11];
-    size_t r, ctl, cc, cs, n2s, tc, ac, ss, sk, sx, nbr, nnb, part, pp, m, nhc;
+    size_t r, ctl, cc, cs, n2s, tc, ac, ss, sk, sx, nbr, nnb, part, pp, m, nhc, box, npt, tens;
     size_t total;
 };
 
@@ -344,6 +377,9 @@ Layout layout_of(const Plan& p)
     L.pp = take(p.pp.size() * sizeof(PairParam));
     L.m = take(p.mass.size() * 8);
     L.nhc = take(NHC_COUNT * 8);
+    L.box = take(BOX_COUNT * 8);
+    L.npt = take(NPT_COUNT * 8);
+    L.tens = take((size_t)((p.npad + 255) / 256) * TENS_COUNT * 8);
     L.total = off;
     return L;
 }
@@ -397,6 +433,12 @@ pis_status pis::alloc_system(SysHost& h)
     d.mass = (const double*)(b + o_m);
     d.nhc = (double*)(b + Lo.nhc);
     d.nhc_on = 0;
+    d.box = (double*)(b + Lo.box);
+    d.npt = (double*)(b + Lo.npt);
+    d.tens = (double*)(b + Lo.tens);
+    d.npt_on = 0;
+    d.tri = p.tri ? 1 : 0;
+    d.nblk = (p.npad + 255) / 256;
     d.thermo = nullptr;
     d.thermo_cap = 0;
     for (int k = 0; k < 3; ++k) {
@@ -443,6 +485,13 @@ pis_status pis::alloc_system(SysHost& h)
     PIS_HIP(hipMemcpy(d.tile_cell0, tc.data(), tc.size() * 4, hipMemcpyHostToDevice));
     PIS_HIP(hipMemcpy((void*)d.pp, p.pp.data(), p.pp.size() * sizeof(PairParam), hipMemcpyHostToDevice));
     PIS_HIP(hipMemcpy((void*)d.mass, p.mass.data(), p.mass.size() * 8, hipMemcpyHostToDevice));
+    double bx[BOX_COUNT] = {0.0};
+    for (int k = 0; k < 9; ++k) {
+        bx[BOX_H + k] = p.h[k];
+        bx[BOX_HINV + k] = p.hinv[k];
+    }
+    bx[BOX_VOL] = p.volume;
+    PIS_HIP(hipMemcpy(d.box, bx, sizeof(bx), hipMemcpyHostToDevice));
     return PIS_OK;
 }
 
@@ -689,8 +738,9 @@ pis_status pis::enqueue_classes(pis_ctx* c, bool build, int force)
     }
     for (const auto& q : c->classes) {
         hipStream_t st = q.lane == 0 ? c->stream : c->side[q.lane - 1];
-        if (build) launch_build_list(st, c->d_sys, c->d_tile_work + q.first, q.count, q.scap);
-        if (force >= 0) launch_force(st, c->d_sys, c->d_tile_work + q.first, q.count, q.scap, c->mixed, force);
+        if (build) launch_build_list(st, c->d_sys, c->d_tile_work + q.first, q.count, q.scap, ctx_kinds(c));
+        if (force >= 0)
+            launch_force(st, c->d_sys, c->d_tile_work + q.first, q.count, q.scap, c->mixed, force, ctx_kinds(c));
     }
     for (int l = 1; l < nl; ++l) {
         PIS_HIP(hipEventRecord(c->join[l - 1], c->side[l - 1]));
@@ -709,6 +759,9 @@ pis_status pis::run_rebuild(pis_ctx* c)
 {
     PIS_TRY(enqueue_rebuild(c));
     launch_flip(c->stream, c->d_sys, (int)c->sys.size());
+    // the other buffer set still has the previous order's types and original indices, and
+    // the force kernel rewrites them there only on rebuild steps
+    launch_sync_labels(c->stream, c->d_sys, c->d_atom_work, c->n_atom_work);
     PIS_HIP(hipGetLastError());
     return PIS_OK;
 }
@@ -721,6 +774,9 @@ pis_status pis::check_flags(pis_ctx* c, int only)
         PIS_TRY(read_ctl(c, k, ctl));
         if (ctl[CTL_OVF_STAGE])
             return fail(PIS_ERR_CAPACITY, "system " + std::to_string(k) + ": LDS staging capacity exceeded");
+        if (ctl[CTL_BOX])
+            return fail(PIS_ERR_CAPACITY, "system " + std::to_string(k) +
+                                              ": the box deformed below the planned cell grid (rcut + skin)");
         if (ctl[CTL_OVF_NB])
             return fail(PIS_ERR_CAPACITY, "system " + std::to_string(k) + ": neighbour capacity " +
                                               std::to_string(c->sys[k].plan.maxnb) + " exceeded (" +
@@ -949,18 +1005,20 @@ pis_status pis::enqueue_steps(pis_ctx* c, uint64_t nsteps, bool record, bool nvt
 // nsteps of verlet_step_nve (nvt = false) or verlet_step_nvt_nhc (nvt = true, chain set
 // by pis_nvt_init) for every system, state resident on the device.
 static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_thermo* out, bool nvt = false,
-                            uint64_t ramp_step0 = 0, uint64_t ramp_total = 0)
+                            uint64_t ramp_step0 = 0, uint64_t ramp_total = 0, bool npt = false)
 {
     if (!c) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
     if (!dts) return fail(PIS_ERR_INVALID_ARGUMENT, "null timestep array");
     if (nsteps == 0) return PIS_OK;
     if (nsteps > (uint64_t)1 << 30) return fail(PIS_ERR_INVALID_ARGUMENT, "too many steps");
     if (nvt && !c->nhc_ready) return fail(PIS_ERR_INVALID_ARGUMENT, "NVT needs pis_nvt_init first");
+    if (npt && !c->npt_ready) return fail(PIS_ERR_INVALID_ARGUMENT, "NPT needs pis_npt_init first");
     for (size_t k = 0; k < c->sys.size(); ++k) {
         if (!(dts[k] > 0.0) || !std::isfinite(dts[k])) return fail(PIS_ERR_INVALID_ARGUMENT, "dt must be > 0");
         c->sys[k].dev.dt = dts[k];
         c->sys[k].dev.dt2 = dts[k] * dts[k];  // dt.powi(2)
         c->sys[k].dev.nhc_on = nvt ? 1 : 0;
+        c->sys[k].dev.npt_on = npt ? 1 : 0;
     }
     const int cap = (int)std::min<uint64_t>(nsteps, (uint64_t)1 << 30);
     PIS_TRY(ensure_thermo(c, out ? cap : 1));
@@ -974,7 +1032,21 @@ static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_
             return fail(PIS_ERR_INVALID_ARGUMENT, "the steps run past the end of the temperature ramp");
         launch_nhc_run(c->stream, c->d_sys, ns, ramp_step0, ramp_total);
     }
-    PIS_TRY(enqueue_steps(c, nsteps, out != nullptr, nvt));
+    if (npt) {
+        // verlet_step_npt_mtk (potential.rs:112-135): barostat and box update, the NVT step
+        // on the scaled state (its list rebuilt for the new box), the second scaling and
+        // momentum update; direct launches, step by step
+        const int rec = out != nullptr ? 1 : 0;
+        for (uint64_t t = 0; t < nsteps; ++t) {
+            launch_npt(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work, 0, rec);
+            launch_npt(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work, 1, rec);
+            PIS_TRY(enqueue_one(c, t == 0, out != nullptr, true, false));
+            launch_npt(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work, 2, rec);
+            if ((t & 63) == 0) PIS_HIP(hipGetLastError());
+        }
+    } else {
+        PIS_TRY(enqueue_steps(c, nsteps, out != nullptr, nvt));
+    }
     PIS_HIP(hipGetLastError());
     PIS_HIP(hipStreamSynchronize(c->stream));
     c->steps += nsteps;
@@ -1013,6 +1085,51 @@ static pis_status nvt_init(pis_ctx* c, const pis_nhc_params* p)
     }
     PIS_HIP(hipStreamSynchronize(c->stream));
     c->nhc_ready = true;
+    return PIS_OK;
+}
+
+// MTKBarostat::new via new_from_args (npt.rs:24-42, 66-87) for every system, together with
+// the chain: target = p0 I, momentum 0, w = 3N KB T_start tau^2 with the constant
+// KB = 0.0083144621 (npt.rs:32; not the system's kb).
+static pis_status npt_init(pis_ctx* c, const pis_nhc_params* chain, const pis_mtk_params* mtk)
+{
+    if (!c || !chain || !mtk) return fail(PIS_ERR_INVALID_ARGUMENT, "null argument");
+    if (!(mtk->tau > 0.0) || !std::isfinite(mtk->tau) || !std::isfinite(mtk->p_start))
+        return fail(PIS_ERR_INVALID_ARGUMENT, "need a finite target pressure and tau > 0");
+    if (mtk->reserved[0] != 0.0 || mtk->reserved[1] != 0.0)
+        return fail(PIS_ERR_INVALID_ARGUMENT, "pis_mtk_params.reserved must be 0");
+    PIS_TRY(nvt_init(c, chain));
+    bool regraph = false;
+    for (auto& h : c->sys) {
+        // the box becomes general with the first step: raw staging with the general-box
+        // image, bands for positions up to 1.25x the current extent (the box may grow)
+        Plan& p = h.plan;
+        double ext = 0.0;
+        for (int r = 0; r < 3; ++r) ext = std::max(ext, std::fabs(p.h[r]) + std::fabs(p.h[3 + r]) + std::fabs(p.h[6 + r]));
+        if (!p.tri) regraph = true;
+        p.tri = true;
+        p.qmax = std::max(p.qmax, 1.25 * ext);
+        if (p.mode == MODE_TILE_REL) p.mode = MODE_RAW_MI;
+        set_bands(&p);
+        h.dev.tri = 1;
+        h.dev.mode = p.mode;
+        h.dev.wslack = p.wslack;
+        PIS_HIP(hipMemcpyAsync((void*)h.dev.pp, p.pp.data(), p.pp.size() * sizeof(PairParam), hipMemcpyHostToDevice,
+                               c->stream));
+        double st[NPT_COUNT] = {0.0};
+        for (int k = 0; k < 3; ++k) st[NPT_TARGET + 4 * k] = mtk->p_start;
+        st[NPT_W] = ((double)(3 * h.plan.n)) * 0.0083144621 * chain->t_start * (mtk->tau * mtk->tau);
+        st[NPT_RCL] = h.plan.rcl * (1.0 + 1e-12);
+        PIS_HIP(hipMemcpyAsync(h.dev.npt, st, sizeof(st), hipMemcpyHostToDevice, c->stream));
+    }
+    PIS_HIP(hipStreamSynchronize(c->stream));
+    if (regraph)  // captured step graphs hold the orthorhombic kernel instantiations
+        for (auto& g : c->graph) {
+            if (g) PIS_HIP(hipGraphExecDestroy(g));
+            g = nullptr;
+        }
+    PIS_TRY(upload_sys(c));
+    c->npt_ready = true;
     return PIS_OK;
 }
 
@@ -1181,6 +1298,33 @@ pis_status pis_batch_verlet_step_nvt_ramp(pis_ctx* ctx, const double* dts, uint6
 {
     if (ramp_total == 0) return fail(PIS_ERR_INVALID_ARGUMENT, "ramp_total must be > 0");
     return batch_run(ctx, dts, nsteps, thermo_out, true, ramp_step0, ramp_total);
+}
+
+pis_status pis_npt_init(pis_ctx* ctx, const pis_nhc_params* chain, const pis_mtk_params* barostat)
+{
+    return npt_init(ctx, chain, barostat);
+}
+
+pis_status pis_batch_verlet_step_npt(pis_ctx* ctx, const double* dts, uint64_t nsteps, uint64_t ramp_step0,
+                                     uint64_t ramp_total, pis_thermo* thermo_out)
+{
+    if (ramp_total == 0) ramp_total = nsteps;
+    return batch_run(ctx, dts, nsteps, thermo_out, true, ramp_step0, ramp_total, true);
+}
+
+pis_status pis_npt_get_box(pis_ctx* ctx, uint32_t sys, double* h_out, double* momentum_out)
+{
+    if (!ctx) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
+    if (sys >= ctx->sys.size()) return fail(PIS_ERR_INVALID_ARGUMENT, "system index out of range");
+    double bx[BOX_COUNT], st[NPT_COUNT];
+    PIS_HIP(hipMemcpyAsync(bx, ctx->sys[sys].dev.box, sizeof(bx), hipMemcpyDeviceToHost, ctx->stream));
+    PIS_HIP(hipMemcpyAsync(st, ctx->sys[sys].dev.npt, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+    PIS_HIP(hipStreamSynchronize(ctx->stream));
+    if (h_out)
+        for (int k = 0; k < 9; ++k) h_out[k] = bx[BOX_H + k];
+    if (momentum_out)
+        for (int k = 0; k < 9; ++k) momentum_out[k] = st[NPT_MOM + k];
+    return PIS_OK;
 }
 
 pis_status pis_get_neighbor_list(pis_ctx* ctx, uint64_t* offsets, uint32_t* idx, uint64_t cap, uint64_t* total)

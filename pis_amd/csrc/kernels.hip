@@ -100,6 +100,80 @@ __device__ __forceinline__ double mi_ref(double d, double L, double Linv)
     return L * s;
 }
 
+// nalgebra Matrix3 * Vector3 (gemv, column by column): y = m[:,0] x0, then y = m[:,c] xc + y,
+// each product and sum rounded (the reference never fuses).  m is column-major.
+__device__ __forceinline__ void gemv3(const double* m, double x0, double x1, double x2, double& y0, double& y1,
+                                      double& y2)
+{
+    double t0 = m[0] * x0, t1 = m[1] * x0, t2 = m[2] * x0;
+    t0 = m[3] * x1 + t0;
+    t1 = m[4] * x1 + t1;
+    t2 = m[5] * x1 + t2;
+    t0 = m[6] * x2 + t0;
+    t1 = m[7] * x2 + t1;
+    t2 = m[8] * x2 + t2;
+    y0 = t0;
+    y1 = t1;
+    y2 = t2;
+}
+
+// apply_boundary_conditions_dis (simulation_box.rs:17-27) for a general box, bit for bit:
+// s = h^-1 d, s -= round(s), d = h s.  box: h and h^-1 (BOX_H, BOX_HINV).
+__device__ __forceinline__ void mi_tri(const double* box, double& dx, double& dy, double& dz)
+{
+    double s0, s1, s2;
+    gemv3(box + BOX_HINV, dx, dy, dz, s0, s1, s2);
+    s0 = s0 - round(s0);
+    s1 = s1 - round(s1);
+    s2 = s2 - round(s2);
+    gemv3(box + BOX_H, s0, s1, s2, dx, dy, dz);
+}
+
+// apply_boundary_conditions_pos (simulation_box.rs:29-42) for a general box:
+// s = h^-1 r, s -= floor(s), r = h s.
+__device__ __forceinline__ void wrap_tri(const double* box, double& x, double& y, double& z)
+{
+    double s0, s1, s2;
+    gemv3(box + BOX_HINV, x, y, z, s0, s1, s2);
+    s0 = s0 - floor(s0);
+    s1 = s1 - floor(s1);
+    s2 = s2 - floor(s2);
+    gemv3(box + BOX_H, s0, s1, s2, x, y, z);
+}
+
+// FP32 minimum image of a general box for the list build's staged coordinates (any
+// rounding: the list band covers it).  h32 / hi32: h and h^-1, column-major.
+__device__ __forceinline__ void mi_tri32(const float* h32, const float* hi32, float& dx, float& dy, float& dz)
+{
+    float s0 = fmaf(hi32[6], dz, fmaf(hi32[3], dy, hi32[0] * dx));
+    float s1 = fmaf(hi32[7], dz, fmaf(hi32[4], dy, hi32[1] * dx));
+    float s2 = fmaf(hi32[8], dz, fmaf(hi32[5], dy, hi32[2] * dx));
+    s0 -= rintf(s0);
+    s1 -= rintf(s1);
+    s2 -= rintf(s2);
+    dx = fmaf(h32[6], s2, fmaf(h32[3], s1, h32[0] * s0));
+    dy = fmaf(h32[7], s2, fmaf(h32[4], s1, h32[1] * s0));
+    dz = fmaf(h32[8], s2, fmaf(h32[5], s1, h32[2] * s0));
+}
+
+// The same in FP64 for the force kernel's staged raw positions (any rounding: outside
+// the rounding band the cutoff decision does not depend on it, inside the band the
+// reference predicate is evaluated exactly).
+__device__ __forceinline__ void mi_tri_fast(const double* box, double& dx, double& dy, double& dz)
+{
+    const double* hi = box + BOX_HINV;
+    const double* h = box + BOX_H;
+    double s0 = fma(hi[6], dz, fma(hi[3], dy, hi[0] * dx));
+    double s1 = fma(hi[7], dz, fma(hi[4], dy, hi[1] * dx));
+    double s2 = fma(hi[8], dz, fma(hi[5], dy, hi[2] * dx));
+    s0 -= rint(s0);
+    s1 -= rint(s1);
+    s2 -= rint(s2);
+    dx = fma(h[6], s2, fma(h[3], s1, h[0] * s0));
+    dy = fma(h[7], s2, fma(h[4], s1, h[1] * s0));
+    dz = fma(h[8], s2, fma(h[5], s1, h[2] * s0));
+}
+
 // Exclusive scan over the block (blockDim.x a multiple of 64, <= 1024).
 __device__ int block_excl_scan(int v, int* tmp, int* total)
 {
@@ -388,19 +462,28 @@ __global__ __launch_bounds__(kAtomThreads) void k_integrate_pos(const SysDev* __
     double y = gbl(st.y)[a] + (vy * dt + (gbl(st.fy)[a] / m) * 0.5 * dt2);
     double z = gbl(st.z)[a] + (vz * dt + (gbl(st.fz)[a] / m) * 0.5 * dt2);
     // apply_boundary_conditions_pos (simulation_box.rs:29-42): s = h^-1 r; s -= floor(s); r = h s
-    double sx = x * s.Linv[0], sy = y * s.Linv[1], sz = z * s.Linv[2];
-    sx = sx - floor(sx);
-    sy = sy - floor(sy);
-    sz = sz - floor(sz);
-    x = s.L[0] * sx;
-    y = s.L[1] * sy;
-    z = s.L[2] * sz;
+    double dx, dy, dz;
+    if (s.tri) {
+        wrap_tri(s.box, x, y, z);
+        dx = x - gbl(s.rx)[a];
+        dy = y - gbl(s.ry)[a];
+        dz = z - gbl(s.rz)[a];
+        mi_tri_fast(s.box, dx, dy, dz);
+    } else {
+        double sx = x * s.Linv[0], sy = y * s.Linv[1], sz = z * s.Linv[2];
+        sx = sx - floor(sx);
+        sy = sy - floor(sy);
+        sz = sz - floor(sz);
+        x = s.L[0] * sx;
+        y = s.L[1] * sy;
+        z = s.L[2] * sz;
+        dx = mi_fast(x - gbl(s.rx)[a], s.L[0], s.Linv[0]);
+        dy = mi_fast(y - gbl(s.ry)[a], s.L[1], s.Linv[1]);
+        dz = mi_fast(z - gbl(s.rz)[a], s.L[2], s.Linv[2]);
+    }
     gbl(st.x)[a] = x;
     gbl(st.y)[a] = y;
     gbl(st.z)[a] = z;
-    const double dx = mi_fast(x - gbl(s.rx)[a], s.L[0], s.Linv[0]);
-    const double dy = mi_fast(y - gbl(s.ry)[a], s.L[1], s.Linv[1]);
-    const double dz = mi_fast(z - gbl(s.rz)[a], s.L[2], s.Linv[2]);
     const double d2 = dx * dx + dy * dy + dz * dz;
     if (!(d2 <= s.half_skin2)) {  // also catches NaN
         atomicOr(&s.ctl[CTL_REBUILD], 1);
@@ -425,11 +508,22 @@ __global__ __launch_bounds__(kAtomThreads) void k_bin(const SysDev* __restrict__
         if (act) {
             const State st = s.st[__builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR])];
             // rcut_cells (neighbour_list.rs:17-34): c = floor(s n) as usize (saturating), % n
-            const double r[3] = {gbl(st.x)[a], gbl(st.y)[a], gbl(st.z)[a]};
+            double r[3] = {gbl(st.x)[a], gbl(st.y)[a], gbl(st.z)[a]};
+            if (s.tri) {  // fractional coordinates s = h^-1 r
+                double f0, f1, f2;
+                gemv3(s.box + BOX_HINV, r[0], r[1], r[2], f0, f1, f2);
+                r[0] = f0;
+                r[1] = f1;
+                r[2] = f2;
+            } else {
+                r[0] = r[0] * s.Linv[0];
+                r[1] = r[1] * s.Linv[1];
+                r[2] = r[2] * s.Linv[2];
+            }
             int c[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const double f = floor((r[d] * s.Linv[d]) * (double)s.nc[d]);
+                const double f = floor(r[d] * (double)s.nc[d]);
                 const int v = f > 0.0 ? (f < 2.0e9 ? (int)f : 2000000000) : 0;
                 c[d] = v % s.nc[d];
             }
@@ -609,12 +703,18 @@ __device__ __forceinline__ int slot_atom(const int* pref, const int* ecell, int 
 // The reference's predicate for one pair, bit for bit: rij = r_j - r_i from the raw
 // positions, apply_boundary_conditions_dis with round(), r2 = (x^2+y^2)+z^2 without
 // fusion, accepted when r2 <= r2max (<=> norm() <= rcut).
+template <bool TRI>
 __device__ __forceinline__ bool exact_within(const SysDev& s, const State& st, int a, double xi,
                                              double yi, double zi, double r2max)
 {
-    const double dx = mi_ref(gbl(st.x)[a] - xi, s.L[0], s.Linv[0]);
-    const double dy = mi_ref(gbl(st.y)[a] - yi, s.L[1], s.Linv[1]);
-    const double dz = mi_ref(gbl(st.z)[a] - zi, s.L[2], s.Linv[2]);
+    double dx = gbl(st.x)[a] - xi, dy = gbl(st.y)[a] - yi, dz = gbl(st.z)[a] - zi;
+    if (TRI) {
+        mi_tri(s.box, dx, dy, dz);
+    } else {
+        dx = mi_ref(dx, s.L[0], s.Linv[0]);
+        dy = mi_ref(dy, s.L[1], s.Linv[1]);
+        dz = mi_ref(dz, s.L[2], s.Linv[2]);
+    }
     return (dx * dx + dy * dy) + dz * dz <= r2max;
 }
 
@@ -641,7 +741,7 @@ __device__ __forceinline__ int wave_max(int v)
 // shift register that leaves as one uint4 list group per 8 entries.  r2 from the staged
 // coordinates decides every pair outside a +-band around the cutoff; inside the band the
 // bit-exact reference predicate (exact_within, raw positions) decides.
-template <int SCAP, bool PAIRMI, bool MULTI>
+template <int SCAP, bool PAIRMI, bool MULTI, bool TRI>
 __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, const TileGeom& g,
                                           const float* Px, const float* Py, const float* Pz,
                                           const uint8_t* Pt, const int* pref, const int* ecell,
@@ -670,6 +770,14 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
     const float qx32 = (float)qx, qy32 = (float)qy, qz32 = (float)qz;
     const float L32[3] = {(float)s.L[0], (float)s.L[1], (float)s.L[2]};
     const float Li32[3] = {(float)s.Linv[0], (float)s.Linv[1], (float)s.Linv[2]};
+    float H32[9], HI32[9];  // general box (TRI): h and h^-1 in FP32
+    if (TRI) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            H32[q] = (float)s.box[BOX_H + q];
+            HI32[q] = (float)s.box[BOX_HINV + q];
+        }
+    }
     // FP32 list-band edges of this lane's type against each type, held in registers
     float tlo[kMaxTypes], thi[kMaxTypes];
     double reach2 = -1.0;  // no pair beyond this r^2 can enter the list
@@ -750,7 +858,9 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
                     }
 #pragma unroll
                     for (int j = 0; j < U; ++j) {
-                        if (PAIRMI) {
+                        if (PAIRMI && TRI) {
+                            mi_tri32(H32, HI32, dx[j], dy[j], dz[j]);
+                        } else if (PAIRMI) {
                             dx[j] = mi_fast32(dx[j], L32[0], Li32[0]);
                             dy[j] = mi_fast32(dy[j], L32[1], Li32[1]);
                             dz[j] = mi_fast32(dz[j], L32[2], Li32[2]);
@@ -800,7 +910,7 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
                         bm &= bm - 1u;
                         const int sl = sb + kk;
                         const PairParam& p = ppl[MULTI ? ti * nt + Pt[sl] : 0];
-                        if (exact_within(s, st, slot_atom(pref, ecell, g.ne, sl), xi, yi, zi, p.r2_list)) {
+                        if (exact_within<TRI>(s, st, slot_atom(pref, ecell, g.ne, sl), xi, yi, zi, p.r2_list)) {
                             if (kk < 32)
                                 mlo |= 1u << kk;
                             else
@@ -843,7 +953,7 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
     return n_real;
 }
 
-template <int SCAP>
+template <int SCAP, bool TRI>
 __global__ __launch_bounds__(kTileThreads) void k_build_list(const SysDev* __restrict__ sys,
                                                               const WorkItem* __restrict__ tiles,
                                                               int ntw)
@@ -862,6 +972,7 @@ __global__ __launch_bounds__(kTileThreads) void k_build_list(const SysDev* __res
         const WorkItem w = tiles[xcd_remap(b, ntw)];
         const SysDev& s = sys[w.sys];
         if (!s.ctl[CTL_REBUILD]) continue;
+        if ((s.tri != 0) != TRI) continue;  // the other instantiation builds this tile
         const State st = s.st[__builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR]) ^ 1];
         const TileGeom g = tile_geom(s, w.idx);
         const int nt = s.ntypes;
@@ -889,12 +1000,15 @@ __global__ __launch_bounds__(kTileThreads) void k_build_list(const SysDev* __res
             const int a = base + lane;
             const bool active = a < a1;
             int n;
-            if (nt == 1)
-                n = rel ? build_atom<SCAP, false, false>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz)
-                        : build_atom<SCAP, true, false>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz);
+            if (TRI)  // a general box always stages raw positions (per-pair image)
+                n = nt == 1 ? build_atom<SCAP, true, false, true>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz)
+                            : build_atom<SCAP, true, true, true>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz);
+            else if (nt == 1)
+                n = rel ? build_atom<SCAP, false, false, false>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz)
+                        : build_atom<SCAP, true, false, false>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz);
             else
-                n = rel ? build_atom<SCAP, false, true>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz)
-                        : build_atom<SCAP, true, true>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz);
+                n = rel ? build_atom<SCAP, false, true, false>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz)
+                        : build_atom<SCAP, true, true, false>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz);
             mymax = max(mymax, n);
         }
         mymax = wave_max(mymax);
@@ -962,7 +1076,7 @@ struct ExactCtx {
 
 typedef __attribute__((address_space(1))) const u32x4 gu32x4;
 
-template <int SCAP, bool MIXED, bool MULTI, bool PAIRMI>
+template <int SCAP, bool MIXED, bool MULTI, bool PAIRMI, bool TRI>
 __device__ __forceinline__ void atom_forces(const SysDev& s, const double* Px, const double* Py,
                                             const double* Pz, const uint8_t* Pt,
                                             const PairParam* ppl, const PairParam& p0, int a, int ti,
@@ -1001,7 +1115,9 @@ __device__ __forceinline__ void atom_forces(const SysDev& s, const double* Px, c
 #pragma unroll
             for (int e = 0; e < B; ++e) {
                 double dx = X[e] - qx, dy = Y[e] - qy, dz = Z[e] - qz;
-                if (PAIRMI) {
+                if (PAIRMI && TRI) {
+                    mi_tri_fast(s.box, dx, dy, dz);
+                } else if (PAIRMI) {
                     dx = mi_fast(dx, s.L[0], s.Linv[0]);
                     dy = mi_fast(dy, s.L[1], s.Linv[1]);
                     dz = mi_fast(dz, s.L[2], s.Linv[2]);
@@ -1025,7 +1141,7 @@ __device__ __forceinline__ void atom_forces(const SysDev& s, const double* Px, c
                 for (int e = 0; e < B; ++e) {
                     const PairParam& p = MULTI ? ppl[ti * s.ntypes + Pt[off[e] >> 3]] : p0;
                     if (h + e < rem && !IN[e] && R2[e] <= p.f_hi)
-                        IN[e] = exact_within(*ex.s, *ex.st,
+                        IN[e] = exact_within<TRI>(*ex.s, *ex.st,
                                              slot_atom(ex.pref, ex.ecell, ex.ne, (int)(off[e] >> 3)), ex.xi,
                                              ex.yi, ex.zi, p.r2_force);
                 }
@@ -1039,8 +1155,8 @@ __device__ __forceinline__ void atom_forces(const SysDev& s, const double* Px, c
     }
 }
 
-template <int SCAP, bool MIXED>
-__global__ __launch_bounds__(kTileThreads, MIXED ? PIS_FORCE_MIN_WAVES_MIXED : PIS_FORCE_MIN_WAVES_FP64) void k_force(const SysDev* __restrict__ sys,
+template <int SCAP, bool MIXED, bool TRI>
+__global__ __launch_bounds__(kTileThreads, TRI ? 1 : (MIXED ? PIS_FORCE_MIN_WAVES_MIXED : PIS_FORCE_MIN_WAVES_FP64)) void k_force(const SysDev* __restrict__ sys,
                                                          const WorkItem* __restrict__ tiles, int ntw,
                                                          int integrate)
 {
@@ -1054,6 +1170,7 @@ __global__ __launch_bounds__(kTileThreads, MIXED ? PIS_FORCE_MIN_WAVES_MIXED : P
 
     const WorkItem w = tiles[xcd_remap(blockIdx.x, ntw)];
     const SysDev& s = sys[w.sys];
+    if ((s.tri != 0) != TRI) return;                 // the other instantiation takes this tile
     if (integrate && s.ctl[CTL_LEFT] == 0) return;  // frozen system: the whole block leaves
     // read set: the live set, or the freshly sorted one on a rebuild step; a step writes
     // its results into the other set (wt), which becomes live at the end of the step
@@ -1096,16 +1213,21 @@ __global__ __launch_bounds__(kTileThreads, MIXED ? PIS_FORCE_MIN_WAVES_MIXED : P
             }
             Acc acc = {0.0, 0.0, 0.0, 0.0, 0.0};
             const ExactCtx ex = {&s, &st, pref, ecell, g.ne, xi, yi, zi};
-            if (nt == 1) {
-                if (rel)
-                    atom_forces<SCAP, MIXED, false, false>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+            if (TRI) {  // a general box always stages raw positions (per-pair image)
+                if (nt == 1)
+                    atom_forces<SCAP, MIXED, false, true, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
                 else
-                    atom_forces<SCAP, MIXED, false, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+                    atom_forces<SCAP, MIXED, true, true, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+            } else if (nt == 1) {
+                if (rel)
+                    atom_forces<SCAP, MIXED, false, false, false>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+                else
+                    atom_forces<SCAP, MIXED, false, true, false>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
             } else {
                 if (rel)
-                    atom_forces<SCAP, MIXED, true, false>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+                    atom_forces<SCAP, MIXED, true, false, false>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
                 else
-                    atom_forces<SCAP, MIXED, true, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+                    atom_forces<SCAP, MIXED, true, true, false>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
             }
             const double m = s.mass[ti];
             // positions again from memory (L2): not held in registers across the pair loop
@@ -1140,19 +1262,28 @@ __global__ __launch_bounds__(kTileThreads, MIXED ? PIS_FORCE_MIN_WAVES_MIXED : P
                     double x = xa + (vx * dt + (acc.fx / m) * 0.5 * dt2);
                     double y = ya + (vy * dt + (acc.fy / m) * 0.5 * dt2);
                     double z = za + (vz * dt + (acc.fz / m) * 0.5 * dt2);
-                    double sx = x * s.Linv[0], sy = y * s.Linv[1], sz = z * s.Linv[2];
-                    sx = sx - floor(sx);
-                    sy = sy - floor(sy);
-                    sz = sz - floor(sz);
-                    x = s.L[0] * sx;
-                    y = s.L[1] * sy;
-                    z = s.L[2] * sz;
+                    double ddx, ddy, ddz;
+                    if (TRI) {
+                        wrap_tri(s.box, x, y, z);
+                        ddx = x - gbl(s.rx)[a];
+                        ddy = y - gbl(s.ry)[a];
+                        ddz = z - gbl(s.rz)[a];
+                        mi_tri_fast(s.box, ddx, ddy, ddz);
+                    } else {
+                        double sx = x * s.Linv[0], sy = y * s.Linv[1], sz = z * s.Linv[2];
+                        sx = sx - floor(sx);
+                        sy = sy - floor(sy);
+                        sz = sz - floor(sz);
+                        x = s.L[0] * sx;
+                        y = s.L[1] * sy;
+                        z = s.L[2] * sz;
+                        ddx = mi_fast(x - gbl(s.rx)[a], s.L[0], s.Linv[0]);
+                        ddy = mi_fast(y - gbl(s.ry)[a], s.L[1], s.Linv[1]);
+                        ddz = mi_fast(z - gbl(s.rz)[a], s.L[2], s.Linv[2]);
+                    }
                     gbl(wt.x)[a] = x;
                     gbl(wt.y)[a] = y;
                     gbl(wt.z)[a] = z;
-                    const double ddx = mi_fast(x - gbl(s.rx)[a], s.L[0], s.Linv[0]);
-                    const double ddy = mi_fast(y - gbl(s.ry)[a], s.L[1], s.Linv[1]);
-                    const double ddz = mi_fast(z - gbl(s.rz)[a], s.L[2], s.Linv[2]);
                     const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
                     if (!(d2 <= s.half_skin2)) {  // also catches NaN
                         atomicOr(&s.ctl[CTL_REBUILD_NEXT], 1);
@@ -1226,18 +1357,31 @@ __device__ __forceinline__ void sum_partials(const SysDev& s, double* v, double*
     block_sum4(v, rtmp);
 }
 
+__device__ void sum_tens(const SysDev& s, double* v, double* tmp);
+
 // Start of an NVT step (potential.rs:41-45): KE of the current velocities (the last
 // force launch's partials), chain forces and half step, and the velocity scale.
 __global__ __launch_bounds__(1024) void k_nhc_pre(const SysDev* __restrict__ sys)
 {
     __shared__ double rtmp[64];
     const SysDev& s = sys[blockIdx.x];
+    __shared__ double ttmp[16 * TENS_COUNT];
     if (!s.nhc_on || s.ctl[CTL_LEFT] == 0) return;
-    double v[4];
-    sum_partials(s, v, rtmp);
+    double ke;
+    if (s.npt_on) {
+        // NPT: the barostat rescaled the velocities since the last force launch; their KE
+        // was summed per block right after the scaling
+        double u[TENS_COUNT];
+        sum_tens(s, u, ttmp);
+        ke = u[TENS_KE];
+    } else {
+        double v[4];
+        sum_partials(s, v, rtmp);
+        ke = v[1];
+    }
     if (threadIdx.x != 0) return;
     double* c = s.nhc;
-    nhc_forces(c, v[1], s.n);
+    nhc_forces(c, ke, s.n);
     nhc_half_step(c, s.dt);
     c[NHC_SCALE] = exp(-0.5 * s.dt * c[NHC_XI]);
 }
@@ -1249,6 +1393,270 @@ __global__ void k_nhc_run(const SysDev* __restrict__ sys, int nsys, double step0
     if (k >= nsys || !sys[k].nhc_on) return;
     sys[k].nhc[NHC_STEP] = step0;
     sys[k].nhc[NHC_TOTAL] = total;
+}
+
+// ---------------------------------------------------------------------------------
+// MTK barostat (ensemble/npt.rs) around the NVT step (potential.rs:112-135).  3x3
+// matrices are column-major; every operation follows the order of the oracle's
+// restatement (no library calls), so both produce the same bits from the same inputs.
+
+__device__ void mat3_mul(const double* a, const double* b, double* out)
+{
+    double t[9];
+    for (int c = 0; c < 3; ++c) gemv3(a, b[3 * c], b[3 * c + 1], b[3 * c + 2], t[3 * c], t[3 * c + 1], t[3 * c + 2]);
+    for (int k = 0; k < 9; ++k) out[k] = t[k];
+}
+
+// exp(A): A halved until its 1-norm is at most 1/2, Taylor series to degree 18, squared
+// back (Moler & Van Loan 2003, method 3).
+__device__ void mat3_exp(const double* a, double* out)
+{
+    double norm = 0.0;
+    for (int c = 0; c < 3; ++c) {
+        const double col = (fabs(a[3 * c]) + fabs(a[3 * c + 1])) + fabs(a[3 * c + 2]);
+        norm = col > norm ? col : norm;
+    }
+    int sq = 0;
+    double scale = 1.0;
+    while (norm * scale > 0.5 && sq < 64) {
+        scale *= 0.5;
+        ++sq;
+    }
+    double x[9], e[9], t[9];
+    for (int k = 0; k < 9; ++k) {
+        x[k] = a[k] * scale;
+        e[k] = 0.0;
+        t[k] = 0.0;
+    }
+    e[0] = e[4] = e[8] = t[0] = t[4] = t[8] = 1.0;
+    for (int j = 1; j <= 18; ++j) {
+        mat3_mul(t, x, t);
+        for (int k = 0; k < 9; ++k) {
+            t[k] = t[k] / (double)j;
+            e[k] = e[k] + t[k];
+        }
+    }
+    for (int q = 0; q < sq; ++q) mat3_mul(e, e, e);
+    for (int k = 0; k < 9; ++k) out[k] = e[k];
+}
+
+// nalgebra 3x3 try_inverse (cofactors, determinant by first-row expansion) and |det|.
+__device__ double mat3_det(const double* h)
+{
+    const double m11 = h[4] * h[8] - h[5] * h[7];
+    const double m12 = h[1] * h[8] - h[2] * h[7];
+    const double m13 = h[1] * h[5] - h[2] * h[4];
+    return h[0] * m11 - h[3] * m12 + h[6] * m13;
+}
+
+__device__ void mat3_inverse(const double* h, double* hi)
+{
+    const double m11 = h[4] * h[8] - h[5] * h[7];
+    const double m12 = h[1] * h[8] - h[2] * h[7];
+    const double m13 = h[1] * h[5] - h[2] * h[4];
+    const double det = h[0] * m11 - h[3] * m12 + h[6] * m13;
+    hi[0] = m11 / det;
+    hi[3] = -(h[3] * h[8] - h[5] * h[6]) / det;
+    hi[6] = (h[3] * h[7] - h[4] * h[6]) / det;
+    hi[1] = -m12 / det;
+    hi[4] = (h[0] * h[8] - h[2] * h[6]) / det;
+    hi[7] = -(h[0] * h[7] - h[1] * h[6]) / det;
+    hi[2] = m13 / det;
+    hi[5] = -(h[0] * h[5] - h[2] * h[3]) / det;
+    hi[8] = (h[0] * h[4] - h[1] * h[3]) / det;
+}
+
+// math.rs:39-41: (a + a^T) * 0.5
+__device__ void symmetrize3(const double* a, double* out)
+{
+    double t[9];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) t[3 * c + r] = (a[3 * c + r] + a[3 * r + c]) * 0.5;
+    for (int k = 0; k < 9; ++k) out[k] = t[k];
+}
+
+// delta_momentum (npt.rs:44-49): symmetrize((P - P0) * (V * 0.5 * dt)), P = (K + W) / V
+__device__ void mtk_add_delta(const double* sums, double* npt, double vol, double dt)
+{
+    double d[9];
+    const double f = vol * 0.5 * dt;
+    for (int k = 0; k < 9; ++k) d[k] = ((sums[TENS_K + k] + sums[TENS_W + k]) / vol - npt[NPT_TARGET + k]) * f;
+    symmetrize3(d, d);
+    for (int k = 0; k < 9; ++k) npt[NPT_MOM + k] += d[k];
+}
+
+// MTKBarostat::scale (npt.rs:51-58): exp(symmetrize(momentum / w) * factor * dt)
+__device__ void mtk_scale(const double* npt, double dt, double factor, double* out)
+{
+    double eta[9], a[9];
+    for (int k = 0; k < 9; ++k) eta[k] = npt[NPT_MOM + k] / npt[NPT_W];
+    symmetrize3(eta, eta);
+    for (int k = 0; k < 9; ++k) a[k] = eta[k] * factor * dt;
+    mat3_exp(a, out);
+}
+
+// Fixed-order sum of the per-block partials of one system (valid in thread 0).
+__device__ void sum_tens(const SysDev& s, double* v, double* tmp /* [16 waves * TENS_COUNT] */)
+{
+    for (int k = 0; k < TENS_COUNT; ++k) v[k] = 0.0;
+    const __attribute__((address_space(1))) double* T = gbl(static_cast<const double*>(s.tens));
+    for (int b = threadIdx.x; b < s.nblk; b += blockDim.x)
+        for (int k = 0; k < TENS_COUNT; ++k) v[k] += T[(size_t)b * TENS_COUNT + k];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int k = 0; k < TENS_COUNT; ++k) {
+        double x = v[k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if (lane == 0) tmp[wid * TENS_COUNT + k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < TENS_COUNT; ++k) {
+            double t = 0.0;
+            for (int w = 0; w < nw; ++w) t += tmp[w * TENS_COUNT + k];
+            v[k] = t;
+        }
+}
+
+// Per 256-atom block of an NPT system: phase NPT_SUMS sums v v^T (kinetic_tensor, no
+// masses, properties.rs:41-43), r F^T (virial_tensor, properties.rs:45-47) and (1/2) m v^2
+// of the live state; NPT_BEGIN first applies the step's velocity scaling and the box
+// scaling of the positions (transformations.rs:6-15: s = h_old^-1 r, r = h s) and sums the
+// KE only; NPT_END applies the velocity scaling, then sums all three.
+enum { NPT_SUMS = 0, NPT_BEGIN = 1, NPT_END = 2 };
+
+__global__ __launch_bounds__(kAtomThreads) void k_npt_atoms(const SysDev* __restrict__ sys,
+                                                             const WorkItem* __restrict__ work, int phase)
+{
+    __shared__ double tmp[4 * TENS_COUNT];
+    const WorkItem w = work[blockIdx.x];
+    const SysDev& s = sys[w.sys];
+    if (!s.npt_on || s.ctl[CTL_LEFT] == 0) return;
+    const int a = w.idx + threadIdx.x;
+    const State st = s.st[__builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR])];
+    double v[TENS_COUNT];
+    for (int k = 0; k < TENS_COUNT; ++k) v[k] = 0.0;
+    if (a < s.n) {
+        double vx = gbl(st.vx)[a], vy = gbl(st.vy)[a], vz = gbl(st.vz)[a];
+        if (phase != NPT_SUMS) {  // atoms.velocities = &scale * &atoms.velocities
+            gemv3(s.npt + NPT_SCALE, vx, vy, vz, vx, vy, vz);
+            gbl(st.vx)[a] = vx;
+            gbl(st.vy)[a] = vy;
+            gbl(st.vz)[a] = vz;
+        }
+        double x = gbl(st.x)[a], y = gbl(st.y)[a], z = gbl(st.z)[a];
+        if (phase == NPT_BEGIN) {
+            double f0, f1, f2;
+            gemv3(s.npt + NPT_HINV_OLD, x, y, z, f0, f1, f2);
+            gemv3(s.box + BOX_H, f0, f1, f2, x, y, z);
+            gbl(st.x)[a] = x;
+            gbl(st.y)[a] = y;
+            gbl(st.z)[a] = z;
+        }
+        if (phase != NPT_BEGIN) {
+            const double f[3] = {gbl(st.fx)[a], gbl(st.fy)[a], gbl(st.fz)[a]};
+            const double r[3] = {x, y, z}, u[3] = {vx, vy, vz};
+            for (int c = 0; c < 3; ++c)
+                for (int q = 0; q < 3; ++q) {
+                    v[TENS_K + 3 * c + q] = u[q] * u[c];
+                    v[TENS_W + 3 * c + q] = r[q] * f[c];
+                }
+        }
+        const double m = s.mass[gbl(st.type)[a]];
+        v[TENS_KE] = 0.5 * m * ((vx * vx + vy * vy) + vz * vz);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int k = 0; k < TENS_COUNT; ++k) {
+        double x = v[k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if (lane == 0) tmp[wid * TENS_COUNT + k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < TENS_COUNT) {
+        double t = 0.0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += tmp[q * TENS_COUNT + threadIdx.x];
+        gbl(s.tens)[(size_t)(w.idx / kAtomThreads) * TENS_COUNT + threadIdx.x] = t;
+    }
+}
+
+// Start of an NPT step (potential.rs:118-125), one workgroup per system: the momentum
+// update from the pressure tensor of the live state, the step's velocity scaling
+// exp(-dt/2 eta_dot), the box h <- exp(dt eta_dot) h with its inverse and volume; the list
+// is rebuilt for the new box (the reference builds it in every force call).  A box that
+// became thinner than the planned cell grid raises CTL_BOX.
+__global__ __launch_bounds__(256) void k_npt_begin(const SysDev* __restrict__ sys)
+{
+    __shared__ double tmp[4 * TENS_COUNT];
+    const SysDev& s = sys[blockIdx.x];
+    if (!s.npt_on || s.ctl[CTL_LEFT] == 0) return;
+    double v[TENS_COUNT];
+    sum_tens(s, v, tmp);
+    if (threadIdx.x != 0) return;
+    double* npt = s.npt;
+    double* box = s.box;
+    mtk_add_delta(v, npt, box[BOX_VOL], s.dt);
+    mtk_scale(npt, s.dt, -0.5, npt + NPT_SCALE);
+    double sh[9], h[9], hi[9];
+    mtk_scale(npt, s.dt, 1.0, sh);
+    for (int k = 0; k < 9; ++k) npt[NPT_HINV_OLD + k] = box[BOX_HINV + k];
+    mat3_mul(sh, box + BOX_H, h);
+    mat3_inverse(h, hi);
+    const double vol = fabs(mat3_det(h));
+    for (int k = 0; k < 9; ++k) {
+        box[BOX_H + k] = h[k];
+        box[BOX_HINV + k] = hi[k];
+    }
+    box[BOX_VOL] = vol;
+    s.ctl[CTL_REBUILD] = 1;
+    if (s.mode != MODE_WHOLE) {
+        for (int k = 0; k < 3; ++k) {
+            const double* p = h + 3 * ((k + 1) % 3);
+            const double* q = h + 3 * ((k + 2) % 3);
+            const double cx = p[1] * q[2] - p[2] * q[1], cy = p[2] * q[0] - p[0] * q[2], cz = p[0] * q[1] - p[1] * q[0];
+            if (!(vol / sqrt(cx * cx + cy * cy + cz * cz) >= npt[NPT_RCL] * (double)s.nc[k])) s.ctl[CTL_BOX] = 1;
+        }
+    }
+}
+
+// End of an NPT step (potential.rs:129-133) after the velocities were scaled again: the
+// second momentum update; the step's thermo record gets the KE, temperature and pressure
+// of the final velocities and the NPT Hamiltonian (simulation.rs:106-114): PE + KE + chain
+// KE + chain PE (0: eta never moves) + barostat KE (npt.rs:60-62) + barostat PE (npt.rs:64-66).
+__global__ __launch_bounds__(256) void k_npt_end(const SysDev* __restrict__ sys, int record)
+{
+    __shared__ double tmp[4 * TENS_COUNT];
+    const SysDev& s = sys[blockIdx.x];
+    if (!s.npt_on || s.ctl[CTL_LEFT] == 0) return;
+    double v[TENS_COUNT];
+    sum_tens(s, v, tmp);
+    if (threadIdx.x != 0) return;
+    double* npt = s.npt;
+    const double* box = s.box;
+    const double vol = box[BOX_VOL];
+    mtk_add_delta(v, npt, vol, s.dt);
+    const int k = s.ctl[CTL_STEP] - 1;
+    if (!record || k < 0 || k >= s.thermo_cap) return;
+    const double* m = npt + NPT_MOM;
+    double mt[9], mm[9], pt[9], ph[9];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) {
+            mt[3 * c + r] = m[3 * r + c];
+            pt[3 * c + r] = npt[NPT_TARGET + 3 * r + c];
+        }
+    mat3_mul(m, mt, mm);
+    const double bke = ((mm[0] + mm[4]) + mm[8]) / (2.0 * npt[NPT_W]);
+    mat3_mul(pt, box + BOX_H, ph);
+    const double bpe = (ph[0] + ph[4]) + ph[8];
+    const double ke = v[TENS_KE];
+    const double trw = (v[TENS_W + 0] + v[TENS_W + 4]) + v[TENS_W + 8];
+    __attribute__((address_space(1))) double* T = gbl(reinterpret_cast<double*>(s.thermo + k));
+    const double pe = T[0];
+    T[1] = ke;
+    T[2] = pe + ke + nhc_kinetic(s.nhc) + 0.0 + bke + bpe;
+    T[3] = (2.0 * ke) / ((double)(3 * (int64_t)s.n) * s.kb);
+    T[4] = (2.0 * ke + trw) / (3.0 * vol);
+    T[5] = trw;
 }
 
 __global__ __launch_bounds__(1024) void k_reduce(const SysDev* __restrict__ sys, int flip, int record,
@@ -1315,6 +1723,20 @@ __global__ void k_gather_ctl(const SysDev* __restrict__ sys, int nsys, int32_t* 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nsys * CTL_COUNT) return;
     out[i] = sys[i / CTL_COUNT].ctl[i % CTL_COUNT];
+}
+
+// After a host-requested sort the set the next step writes into still holds the previous
+// order's types and original indices: copy them from the live set.
+__global__ __launch_bounds__(kAtomThreads) void k_sync_labels(const SysDev* __restrict__ sys,
+                                                               const WorkItem* __restrict__ work)
+{
+    const WorkItem w = work[blockIdx.x];
+    const SysDev& s = sys[w.sys];
+    const int a = w.idx + threadIdx.x;
+    if (a >= s.n) return;
+    const int cur = __builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR]);
+    gbl(s.st[cur ^ 1].type)[a] = gbl(s.st[cur].type)[a];
+    gbl(s.st[cur ^ 1].orig)[a] = gbl(s.st[cur].orig)[a];
 }
 
 __global__ void k_flip(const SysDev* __restrict__ sys, int nsys)
@@ -1442,31 +1864,49 @@ void launch_rebuild_sort(hipStream_t st, const SysDev* d_sys, int nsys, const Wo
 }
 
 void launch_build_list(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work, int n_tile_work,
-                       int scap)
+                       int scap, int kinds)
 {
     if (n_tile_work <= 0) return;
     const int gt = grid_cap(n_tile_work, PIS_BUILD_GRID_CAP);
 #define PIS_BUILD(S) \
-    hipLaunchKernelGGL(k_build_list<S>, dim3(gt), dim3(kTileThreads), 0, st, d_sys, tile_work, n_tile_work)
-    PIS_SCAP_DISPATCH(scap, PIS_BUILD);
+    hipLaunchKernelGGL((k_build_list<S, false>), dim3(gt), dim3(kTileThreads), 0, st, d_sys, tile_work, n_tile_work)
+#define PIS_BUILD_TRI(S) \
+    hipLaunchKernelGGL((k_build_list<S, true>), dim3(gt), dim3(kTileThreads), 0, st, d_sys, tile_work, n_tile_work)
+    if (kinds & KIND_ORTHO) PIS_SCAP_DISPATCH(scap, PIS_BUILD);
+    if (kinds & KIND_TRI) PIS_SCAP_DISPATCH(scap, PIS_BUILD_TRI);
 #undef PIS_BUILD
+#undef PIS_BUILD_TRI
 }
 
 void launch_force(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work, int n_tile_work,
-                  int scap, bool mixed, int integrate)
+                  int scap, bool mixed, int integrate, int kinds)
 {
     if (n_tile_work <= 0) return;
     const dim3 g(n_tile_work), b(kTileThreads);
 #define PIS_FORCE_M(S) \
-    hipLaunchKernelGGL((k_force<S, true>), g, b, 0, st, d_sys, tile_work, n_tile_work, integrate)
+    hipLaunchKernelGGL((k_force<S, true, false>), g, b, 0, st, d_sys, tile_work, n_tile_work, integrate)
 #define PIS_FORCE_D(S) \
-    hipLaunchKernelGGL((k_force<S, false>), g, b, 0, st, d_sys, tile_work, n_tile_work, integrate)
-    if (mixed)
-        PIS_SCAP_DISPATCH(scap, PIS_FORCE_M);
-    else
-        PIS_SCAP_DISPATCH(scap, PIS_FORCE_D);
+    hipLaunchKernelGGL((k_force<S, false, false>), g, b, 0, st, d_sys, tile_work, n_tile_work, integrate)
+#define PIS_FORCE_MT(S) \
+    hipLaunchKernelGGL((k_force<S, true, true>), g, b, 0, st, d_sys, tile_work, n_tile_work, integrate)
+#define PIS_FORCE_DT(S) \
+    hipLaunchKernelGGL((k_force<S, false, true>), g, b, 0, st, d_sys, tile_work, n_tile_work, integrate)
+    if (kinds & KIND_ORTHO) {
+        if (mixed)
+            PIS_SCAP_DISPATCH(scap, PIS_FORCE_M);
+        else
+            PIS_SCAP_DISPATCH(scap, PIS_FORCE_D);
+    }
+    if (kinds & KIND_TRI) {
+        if (mixed)
+            PIS_SCAP_DISPATCH(scap, PIS_FORCE_MT);
+        else
+            PIS_SCAP_DISPATCH(scap, PIS_FORCE_DT);
+    }
 #undef PIS_FORCE_M
 #undef PIS_FORCE_D
+#undef PIS_FORCE_MT
+#undef PIS_FORCE_DT
 }
 
 void launch_reduce(hipStream_t st, const SysDev* d_sys, int nsys, int flip, int record, int step)
@@ -1489,6 +1929,27 @@ void launch_gather_ctl(hipStream_t st, const SysDev* d_sys, int nsys, int32_t* o
 {
     const int n = nsys * CTL_COUNT;
     hipLaunchKernelGGL(k_gather_ctl, dim3((n + 255) / 256), dim3(256), 0, st, d_sys, nsys, out);
+}
+
+void launch_npt(hipStream_t st, const SysDev* d_sys, int nsys, const WorkItem* atom_work, int n_atom_work,
+                int what, int record)
+{
+    // what: 0 start-of-step sums, 1 start (barostat, box, scaled state), 2 end of the step
+    if (what == 0) {
+        hipLaunchKernelGGL(k_npt_atoms, dim3(n_atom_work), dim3(kAtomThreads), 0, st, d_sys, atom_work, (int)NPT_SUMS);
+    } else if (what == 1) {
+        hipLaunchKernelGGL(k_npt_begin, dim3(nsys), dim3(256), 0, st, d_sys);
+        hipLaunchKernelGGL(k_npt_atoms, dim3(n_atom_work), dim3(kAtomThreads), 0, st, d_sys, atom_work, (int)NPT_BEGIN);
+    } else {
+        hipLaunchKernelGGL(k_npt_atoms, dim3(n_atom_work), dim3(kAtomThreads), 0, st, d_sys, atom_work, (int)NPT_END);
+        hipLaunchKernelGGL(k_npt_end, dim3(nsys), dim3(256), 0, st, d_sys, record);
+    }
+}
+
+void launch_sync_labels(hipStream_t st, const SysDev* d_sys, const WorkItem* atom_work, int n_atom_work)
+{
+    if (n_atom_work <= 0) return;
+    hipLaunchKernelGGL(k_sync_labels, dim3(n_atom_work), dim3(kAtomThreads), 0, st, d_sys, atom_work);
 }
 
 void launch_flip(hipStream_t st, const SysDev* d_sys, int nsys)

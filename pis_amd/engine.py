@@ -168,6 +168,31 @@ class Engine:
             return None
         return thermo_to_array(recs).reshape(nsteps, self.nsys, 7)
 
+    def npt_init(self, t_start: float, t_end: float, tau_t: float, p_start: float, tau_p: float, chain: int = 3):
+        """MTK barostat + Nose-Hoover chain for every system (MTKBarostat::new_from_args,
+        target pressure p_start * I)."""
+        prm = _capi.NHCParams(float(t_start), float(t_end), float(tau_t), int(chain), 0)
+        mtk = _capi.MTKParams(float(p_start), float(tau_p))
+        check(lib().pis_npt_init(self._ptr, C.byref(prm), C.byref(mtk)))
+
+    def run_npt(self, dt, nsteps: int, record: bool = True, ramp=None):
+        """nsteps of verlet_step_npt_mtk for every system; thermo [nsteps, nsys, 7] with
+        etot the NPT Hamiltonian.  ramp as in run_nvt."""
+        dts = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (self.nsys,)))
+        recs = (_capi.Thermo * (nsteps * self.nsys))() if record else None
+        r0, rt = (0, 0) if ramp is None else (int(ramp[0]), int(ramp[1]))
+        check(lib().pis_batch_verlet_step_npt(self._ptr, _dptr(dts), nsteps, r0, rt, recs))
+        if not record:
+            return None
+        return thermo_to_array(recs).reshape(nsteps, self.nsys, 7)
+
+    def npt_box(self, k: int = 0):
+        """(h, momentum) of system k as 3x3 arrays (h's columns are the box vectors)."""
+        h = np.zeros(9)
+        m = np.zeros(9)
+        check(lib().pis_npt_get_box(self._ptr, k, _dptr(h), _dptr(m)))
+        return h.reshape(3, 3).T.copy(), m.reshape(3, 3).T.copy()
+
     def nvt_chain(self, k: int = 0, chain: int = 3):
         """(xi[chain], target temperature) of system k's thermostat chain."""
         xi = np.zeros(8)

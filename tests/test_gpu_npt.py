@@ -1,0 +1,124 @@
+"""GPU: general (tilted) boxes and the NPT ensemble against the oracle.
+
+* forces, energies and the pair set of sheared systems (raw staging with tiles, and the
+  whole-box staging of a small box), FP64 to 1e-10 and mixed to the RMS bar;
+* NVE on a sheared box tracks the oracle (triclinic wrap and skin check);
+* verlet_step_npt_mtk (potential.rs:112-135) run by Simulation::run's NPT loop tracks the
+  oracle's restatement: thermo (with the barostat's energy terms in H) and the box matrix,
+  which becomes general after the first step;
+* a barostat that squeezes the box below the planned cell grid stops with
+  PIS_ERR_CAPACITY instead of missing pairs.
+
+Tilted test systems are made by shearing an orthorhombic liquid together with its box
+(x -> D x, h -> D h), which keeps them periodic without overlaps.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from pis_amd import Engine, PisError, SimulationBox, workloads as W
+from tests.conftest import force_err, force_rms_err
+
+pytestmark = pytest.mark.gpu
+
+SHEAR = np.array([[1.0, 0.06, 0.03], [0.0, 1.0, 0.04], [0.0, 0.0, 1.0]])
+
+
+def sheared(spec, D=SHEAR):
+    h = np.array(spec.box.h).reshape(3, 3).T  # columns are the box vectors
+    spec.box = SimulationBox((D @ h).T.reshape(9))
+    spec.positions = np.ascontiguousarray(spec.positions @ D.T)
+    return spec
+
+
+CASES = {
+    "tiled_2048": lambda: sheared(W.lj_liquid(8, 0.8442, 0.9, seed=61, rcut=2.5, skin=0.3, dt=0.004)),
+    "whole_500": lambda: sheared(W.lj_liquid(5, 0.8442, 0.9, seed=62, rcut=2.5, skin=0.3, dt=0.004)),
+    "ka_tiled": lambda: sheared(W.ka_mixture(4000, 1.2, 1.0, seed=63)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_sheared_box_forces_and_pairs_match_oracle(name):
+    spec = CASES[name]()
+    print(f"{name}: N={spec.n}", flush=True)
+    o = po.OSystem.from_spec(spec)
+    pe_ref = o.compute(po.MGR_BRUTE)
+    f_ref = o.force.copy()
+    st, ix = o.list(rcut_add=spec.skin, brute=True)
+    with Engine(spec, "fp64") as eng:
+        eng.load()
+        off, idx = eng.neighbor_list()
+        pe = float(eng.compute_potential()[0])
+        _, _, f = eng.get_state(0)
+    assert np.array_equal(po.csr_to_sorted_pairs(off, idx), po.csr_to_sorted_pairs(st, ix))
+    assert force_err(f, f_ref) <= 1e-10
+    assert abs(pe - pe_ref) <= 1e-11 * abs(pe_ref)
+    with Engine(spec, "mixed") as eng:
+        eng.load()
+        pe_m = float(eng.compute_potential()[0])
+        _, _, f_m = eng.get_state(0)
+    assert force_rms_err(f_m, f_ref) <= 1e-5
+    assert abs(pe_m - pe_ref) <= 1e-6 * abs(pe_ref)
+
+
+def test_sheared_box_nve_tracks_oracle():
+    spec = CASES["tiled_2048"]()
+    o = po.OSystem.from_spec(spec)
+    ref = o.run_nve(spec.dt, 300, po.MGR_OFFSET, kb=spec.kb)
+    with Engine(spec, "fp64") as eng:
+        eng.load()
+        eng.compute_potential()
+        th = eng.run_nve(spec.dt, 300)[:, 0, :]
+        pos, _, _ = eng.get_state(0)
+    scale = np.abs(ref[:, 0]).max()
+    assert np.abs(th[:, 0] - ref[:, 0]).max() <= 1e-9 * scale
+    assert np.abs(th[:, 1] - ref[:, 1]).max() <= 1e-9 * scale
+    # positions wrapped into the sheared box the same way
+    h = np.array(spec.box.h).reshape(3, 3).T
+    d = (pos - o.pos) @ np.linalg.inv(h).T
+    assert np.abs(d - np.round(d)).max() <= 1e-10 and np.abs(d).max() <= 1e-10
+
+
+NPT_CASES = {
+    # (spec, T0, tau_t, p0, tau_p, steps)
+    "whole_500": (lambda: W.lj_liquid(5, 0.8442, 0.9, seed=64, rcut=2.5, skin=0.3, dt=0.004), 0.9, 0.5, 1.4, 10.0, 100),
+    "tiled_2048": (lambda: W.lj_liquid(8, 0.8442, 0.9, seed=65, rcut=2.5, skin=0.3, dt=0.004), 0.9, 0.5, 1.4, 20.0, 100),
+}
+
+
+@pytest.mark.parametrize("name", sorted(NPT_CASES))
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_npt_tracks_oracle(name, precision):
+    make, t0, tau_t, p0, tau_p, steps = NPT_CASES[name]
+    spec = make()
+    print(f"{name} {precision}: N={spec.n}", flush=True)
+    o = po.OSystem.from_spec(spec)
+    ref, hb = o.run_npt(spec.dt, steps, t0, t0, tau_t, p0, tau_p, 3, kb=spec.kb)
+    with Engine(spec, precision) as eng:
+        eng.load()
+        eng.compute_potential()
+        eng.npt_init(t0, t0, tau_t, p0, tau_p, 3)
+        th = eng.run_npt(spec.dt, steps)[:, 0, :]
+        h, mom = eng.npt_box(0)
+    h_ref = hb[-1].reshape(3, 3).T
+    tol = 1e-8 if precision == "fp64" else 1e-5
+    scale = np.abs(ref[:, 0]).max()
+    for col in (0, 1, 2):  # PE, KE, H
+        assert np.abs(th[:, col] - ref[:, col]).max() <= tol * scale, col
+    assert np.abs(th[:, 4] - ref[:, 4]).max() <= tol * 100 * max(1.0, np.abs(ref[:, 4]).max())
+    assert np.abs(h - h_ref).max() <= tol * np.abs(h_ref).max()
+    # the box became general: off-diagonal terms from the symmetrised pressure tensor
+    assert np.abs(h - np.diag(np.diag(h))).max() > 0.0
+    assert np.allclose(mom, mom.T, rtol=0, atol=1e-12 * max(1.0, np.abs(mom).max()))
+
+
+def test_npt_squeezed_below_the_cell_grid_stops_with_capacity_error():
+    spec = W.lj_liquid(8, 0.8442, 0.9, seed=66, rcut=2.5, skin=0.3, dt=0.004)
+    with Engine(spec, "fp64") as eng:
+        eng.load()
+        eng.compute_potential()
+        eng.npt_init(0.9, 0.9, 0.5, 12.0, 8.0, 3)  # a large target pressure: the box shrinks
+        with pytest.raises(PisError) as e:
+            eng.run_npt(spec.dt, 80)  # the oracle crosses the grid at step 57
+        assert e.value.status == 10

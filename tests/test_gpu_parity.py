@@ -192,3 +192,23 @@ def test_nvt_tracks_oracle_nose_hoover_chain():
     L = spec.box.lengths
     d = (pos - o.pos + L / 2) % L - L / 2
     assert np.abs(d).max() <= 1e-8
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_state_comes_back_in_the_callers_atom_order_after_steps(precision):
+    # two atom types: after set_state's sort the set written by the first step must carry
+    # the sorted types and original indices (else types and read-back order are scrambled)
+    spec = W.ka_mixture(4000, 1.2, 1.0, seed=71)
+    o = po.OSystem.from_spec(spec)
+    ref = o.run_nve(spec.dt, 3, po.MGR_OFFSET, kb=spec.kb)
+    with Engine(spec, precision) as eng:
+        eng.load()
+        eng.compute_potential()
+        th = eng.run_nve(spec.dt, 3)[:, 0, :]
+        pos, vel, _ = eng.get_state(0)
+    L = np.asarray(spec.box.lengths)
+    d = (pos - o.pos + L / 2) % L - L / 2
+    tol = 1e-10 if precision == "fp64" else 1e-6
+    assert np.abs(d).max() <= tol
+    assert np.abs(vel - o.vel).max() <= tol * 10
+    assert abs(th[-1, 0] - ref[-1, 0]) <= 1e-6 * abs(ref[-1, 0])
