@@ -7,7 +7,7 @@ Restates, with the same accepted syntax and the same errors:
   * the data-file reader `run_read_data` (commands.rs:147-384),
   * `System::contextualize` (system.rs:100-175): starting velocities and the lj/cut table,
   * `Simulation::run` (src/simulation.rs:8-116): the per-step thermo line and the dump
-    cadence, with the steps run by the device-resident engine,
+    cadence, with the steps run by the device-resident engine (NVE, NVT and NPT),
   * the trajectory writer `DumpTraj` (src/writers/dump_traj.rs:12-75), numbers printed the
     way Rust's `{}` prints them.
 
@@ -482,8 +482,6 @@ def run(ctx: Context, precision: str = "fp64", skin: float = 0.3, print_fn=print
     from .engine import Engine
 
     ens = ctx.ensemble
-    if ens == "npt":
-        raise PisScriptError("Unsupported", what="fix npt (MTK barostat) is not part of the device path")
     spec = to_spec(ctx, skin)
     nd = max(1, ctx.dump_args.dump_step)
     dumper = DumpTraj(os.path.join(ctx.base_dir, ctx.dump_args.file_name)) if dump else None
@@ -495,14 +493,21 @@ def run(ctx: Context, precision: str = "fp64", skin: float = 0.3, print_fn=print
             dumper.write_step(spec.positions, spec.type_ids, spec.box, 0)
         pe0 = float(eng.compute_potential()[0])
         print_fn(f"0 {rust_float(pe0)}")
+        nh = ctx.nh_chain_args
         if ens == "nvt":
-            nh = ctx.nh_chain_args
             eng.nvt_init(nh["start_temperature"], nh["end_temperature"], nh["tau"], 3)
+        elif ens == "npt":
+            # MTKBarostat::new_from_args (npt.rs:66-87): the chain's start temperature
+            mtk = ctx.mtk_barostat_args
+            eng.npt_init(nh["start_temperature"], nh["end_temperature"], nh["tau"], mtk["start_pressure"],
+                         mtk["tau"], 3)
         done = 0
         while done < ctx.steps:
             k = min(nd - done % nd, ctx.steps - done)
             if ens == "nvt":
                 th = eng.run_nvt(ctx.timestep, k, ramp=(done, ctx.steps))[:, 0, :]
+            elif ens == "npt":
+                th = eng.run_npt(ctx.timestep, k, ramp=(done, ctx.steps))[:, 0, :]
             else:
                 th = eng.run_nve(ctx.timestep, k)[:, 0, :]
             for j in range(k):
@@ -511,9 +516,12 @@ def run(ctx: Context, precision: str = "fp64", skin: float = 0.3, print_fn=print
             done += k
             if dumper is not None and done % nd == 0:
                 pos, _, _ = eng.get_state(0)
+                box = spec.box
+                if ens == "npt":  # the barostat moved the box
+                    box = SimulationBox(eng.npt_box(0)[0].T.reshape(9))
                 if writer is not None:
                     writer.join()
-                writer = threading.Thread(target=dumper.write_step, args=(pos, spec.type_ids, spec.box, done))
+                writer = threading.Thread(target=dumper.write_step, args=(pos, spec.type_ids, box, done))
                 writer.start()
     if writer is not None:
         writer.join()

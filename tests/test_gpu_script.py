@@ -14,9 +14,13 @@ pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-@pytest.mark.parametrize("ensemble", ["nve", "nvt"])
+FIX = {"nve": "", "nvt": "fix t all nvt temp 100.0 120.0 1.0\n",
+       "npt": "fix t all npt temp 100.0 120.0 1.0 iso 0.02 0.02 5.0\n"}
+
+
+@pytest.mark.parametrize("ensemble", ["nve", "nvt", "npt"])
 def test_script_run_matches_oracle(tmp_path, ensemble):
-    fix = "fix t all nvt temp 100.0 120.0 1.0\n" if ensemble == "nvt" else ""
+    fix = FIX[ensemble]
     inp = tmp_path / "in.pis"
     inp.write_text(f"timestep 0.01\nread_data {os.path.join(GOLDEN, 'example.txt')}\n"
                    f"velocity all create 100.0 7\n{fix}dump d all atom 10 traj.lammpstrj\nrun 40\n")
@@ -27,8 +31,12 @@ def test_script_run_matches_oracle(tmp_path, ensemble):
     assert len(lines) == 41 and lines[0].startswith("0 ")
     # the oracle, step by step, with the trajectory at every 10th step
     o = po.OSystem.from_spec(spec)
+    hb = None
     if ensemble == "nvt":
         ref = o.run_nvt(0.01, 40, 100.0, 120.0, 1.0, 3, po.MGR_OFFSET, kb=spec.kb)
+        frames = None
+    elif ensemble == "npt":
+        ref, hb = o.run_npt(0.01, 40, 100.0, 120.0, 1.0, 0.02, 5.0, 3, po.MGR_OFFSET, kb=spec.kb)
         frames = None
     else:
         o.compute(po.MGR_OFFSET)
@@ -54,6 +62,12 @@ def test_script_run_matches_oracle(tmp_path, ensemble):
     for b, step in enumerate((0, 10, 20, 30, 40)):
         head = text[b * block: b * block + 9]
         assert head[0] == "ITEM: TIMESTEP" and head[1] == str(step) and head[3] == str(n)
+        if hb is not None and step:
+            # the barostat's box: BOX BOUNDS print the diagonal of h (dump_traj.rs:44-50)
+            want = hb[step - 1].reshape(3, 3).T
+            for k in range(3):
+                lo, hi = head[5 + k].split()
+                assert lo == "0" and abs(float(hi) - want[k, k]) <= 1e-9 * want[k, k]
         rows = np.array([[float(v) for v in r.split()[2:]] for r in text[b * block + 9:(b + 1) * block]])
         if frames is not None and step in frames:
             L = spec.box.lengths
