@@ -19,6 +19,12 @@
  *                           boxes.  Exact bits for triclinic boxes: parity unpinned.
  *   orc_gemv             -- nalgebra gemv column order; exact for orthorhombic boxes
  *                           (off-diagonal terms are signed zeros), unpinned for tilt.
+ *   orc_mat3_exp         -- this oracle's own method (nalgebra's Pade exp() is not
+ *                           vendored): checked against scipy.linalg.expm; NPT parity
+ *                           with the reference is tolerance-based.
+ *   orc_kinetic_tensor,
+ *   orc_virial_tensor    -- sums in ascending atom order (matrixmultiply's blocked
+ *                           order is not restated): tolerance-based.
  *   everything else      -- pinned by closed forms, brute force and invariants in
  *                           tests/test_oracle.py (listed there per function).
  */
