@@ -146,6 +146,43 @@ def run_per_call(spec, steps):
             "ms_per_step": round(1e3 * sec / steps, 3)}
 
 
+def run_thermostats(spec, nvt_steps, npt_steps):
+    """NVT (Nose-Hoover chain, graph replay) and NPT (MTK barostat around the NVT step; the
+    box becomes general and the list is rebuilt every step, as the reference's force call
+    does) on the same system, mixed precision.  NPT's target is the starting pressure with
+    a heavy barostat (tau_p = 50), so the box stays inside its cell grid."""
+    from pis_amd import Engine
+
+    out = {}
+    T = 0.72
+    if nvt_steps:
+        with Engine(spec, "mixed") as eng:
+            eng.load()
+            eng.compute_potential()
+            eng.nvt_init(T, T, 0.5, 3)
+            eng.run_nvt(spec.dt, 50, record=False)
+            t0 = time.perf_counter()
+            eng.run_nvt(spec.dt, nvt_steps, record=False)
+            sec = time.perf_counter() - t0
+        out["nvt"] = {"value": round(spec.n * nvt_steps / sec, 1), "unit": "atom-steps/s", "steps": nvt_steps,
+                      "ms_per_step": round(1e3 * sec / nvt_steps, 4), "tau_t": 0.5}
+    if npt_steps:
+        with Engine(spec, "mixed") as eng:
+            eng.load()
+            eng.compute_potential()
+            p0 = float(eng.run_nve(spec.dt, 1)[0, 0, 4])
+            eng.npt_init(T, T, 0.5, p0, 50.0, 3)
+            eng.run_npt(spec.dt, 10, record=False)
+            t0 = time.perf_counter()
+            eng.run_npt(spec.dt, npt_steps, record=False)
+            sec = time.perf_counter() - t0
+            h, _ = eng.npt_box(0)
+        out["npt"] = {"value": round(spec.n * npt_steps / sec, 1), "unit": "atom-steps/s", "steps": npt_steps,
+                      "ms_per_step": round(1e3 * sec / npt_steps, 4), "p_target": round(p0, 6), "tau_p": 50.0,
+                      "max_tilt": float(np.abs(h - np.diag(np.diag(h))).max())}
+    return out
+
+
 def run_ensemble(steps, warmup, direct_steps):
     """Batched ensemble: 512 Kob-Andersen systems (256-32,768 atoms, log-uniform from seed 7,
     density 1.2, T* 0.5-1.5) in one context, mixed precision.  Each step is one replay of
@@ -300,6 +337,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--per-call-steps", type=int, default=20,
                     help="steps of the PCIe-inclusive per-call path (0: skip)")
+    ap.add_argument("--nvt-steps", type=int, default=2000, help="steps of the NVT leg (0: skip)")
+    ap.add_argument("--npt-steps", type=int, default=300, help="steps of the NPT leg (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=5000,
                     help="steps of the 512-system batched ensemble (0: skip)")
     ap.add_argument("--serve-jobs", type=int, default=100,
@@ -333,6 +372,13 @@ def main():
         log(f"per-call path (state over PCIe every step): {args.per_call_steps} steps")
         per_call = run_per_call(spec, args.per_call_steps)
         log(f"per-call: {per_call['value']:.4e} atom-steps/s")
+
+    thermostats = None
+    if args.nvt_steps or args.npt_steps:
+        log(f"NVT {args.nvt_steps} steps, NPT {args.npt_steps} steps (mixed)")
+        thermostats = run_thermostats(spec, args.nvt_steps, args.npt_steps)
+        for k, v in thermostats.items():
+            log(f"{k}: {v['value']:.4e} atom-steps/s, {v['ms_per_step']} ms/step")
 
     ensemble = None
     if args.ensemble_steps:
@@ -438,6 +484,7 @@ def main():
         "step_bytes_per_atom": round(step_bytes_per_atom(nbar), 1),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "thermostat_barostat": thermostats,
         "batched_ensemble": ensemble,
         "pcie_inclusive_per_call": per_call,
         "job_latency": serving,
