@@ -51,7 +51,8 @@ typedef enum pis_status {
     PIS_ERR_NO_DEVICE = 9,
     PIS_ERR_CAPACITY = 10, /* neighbour/staging capacity still exceeded after regrowth */
     PIS_ERR_NOT_FOUND = 11,
-    PIS_ERR_TIMEOUT = 12
+    PIS_ERR_TIMEOUT = 12,
+    PIS_ERR_IO = 13 /* PisError::DumpWriteError (errors.rs:50-55) */
 } pis_status;
 
 typedef enum pis_precision {
@@ -282,6 +283,20 @@ pis_status pis_server_wait(pis_server* srv, uint64_t job_id, double timeout_s, p
 pis_status pis_server_get_stats(pis_server* srv, pis_server_stats* out);
 /* Stops the worker; queued and running jobs end with an error. */
 pis_status pis_server_destroy(pis_server* srv);
+
+/* ---- Trajectory writer (host only) ------------------------------------------------
+ *
+ * DumpTraj::write_step (src/writers/dump_traj.rs:33-74): appends one LAMMPS text block to
+ * the open file descriptor fd -- "ITEM: TIMESTEP", step, "ITEM: NUMBER OF ATOMS", n_atoms,
+ * "ITEM: BOX BOUNDS pp pp pp", "0 h(k,k)" for k = 0..2 (h column-major, 9 doubles),
+ * "ITEM: ATOMS id type x y z" and "i+1 type x y z" per atom (pos AoS xyz, original order).
+ * Numbers are printed as Rust's `{}` prints them (shortest round-trip digits, positional).
+ * threads <= 0: one formatting thread per hardware thread.  PIS_ERR_IO if a write fails. */
+pis_status pis_dump_write_block(int fd, uint64_t step, uint64_t n_atoms, const double* pos,
+                                const uint32_t* type_ids, const double* h, int32_t threads);
+/* One f64 as Rust's `{}` prints it (at most 330 characters plus the terminating NUL);
+ * *len receives the length.  PIS_ERR_INVALID_ARGUMENT if cap is too small. */
+pis_status pis_format_f64(double x, char* out, size_t cap, size_t* len);
 
 #ifdef __cplusplus
 }

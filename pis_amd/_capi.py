@@ -26,6 +26,7 @@ STATUS_NAMES = {
     10: "PIS_ERR_CAPACITY",
     11: "PIS_ERR_NOT_FOUND",
     12: "PIS_ERR_TIMEOUT",
+    13: "PIS_ERR_IO",
 }
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
@@ -194,6 +195,9 @@ SIGNATURES = {
     "pis_server_wait": (C.c_int, [_P, C.c_uint64, C.c_double, C.POINTER(JobResult), _DP, _DP, _DP]),
     "pis_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
     "pis_server_destroy": (C.c_int, [_P]),
+    "pis_dump_write_block": (C.c_int, [C.c_int, C.c_uint64, C.c_uint64, _DP, C.POINTER(C.c_uint32), _DP,
+                                       C.c_int32]),
+    "pis_format_f64": (C.c_int, [C.c_double, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
 }
 
 _lib = None

@@ -1158,6 +1158,7 @@ const char* pis_status_string(pis_status s)
     case PIS_ERR_CAPACITY: return "capacity exceeded";
     case PIS_ERR_NOT_FOUND: return "not found";
     case PIS_ERR_TIMEOUT: return "timed out";
+    case PIS_ERR_IO: return "trajectory write failed";
     }
     return "unknown status";
 }

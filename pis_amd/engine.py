@@ -134,6 +134,12 @@ class Engine:
         check(lib().pis_sys_get_state(self._ptr, k, _dptr(pos), _dptr(vel), _dptr(force)))
         return pos, vel, force
 
+    def get_positions(self, k: int = 0) -> np.ndarray:
+        """Positions of system k alone (original order), for trajectory frames."""
+        pos = np.empty((self.specs[k].n, 3))
+        check(lib().pis_sys_get_state(self._ptr, k, _dptr(pos), None, None))
+        return pos
+
     def compute_potential(self) -> np.ndarray:
         pe = np.empty(self.nsys)
         check(lib().pis_batch_compute_potential(self._ptr, _dptr(pe)))

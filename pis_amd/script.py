@@ -446,27 +446,32 @@ def rust_float(x: float) -> str:
 
 
 class DumpTraj:
-    """Text trajectory, one block per written step (dump_traj.rs:33-74)."""
+    """Text trajectory, one block per written step (dump_traj.rs:33-74).  The block is
+    formatted and written by libpis_hip's host writer (pis_dump_write_block), which prints
+    numbers exactly as `rust_float` does, on several threads."""
 
-    def __init__(self, path: str):
+    def __init__(self, path: str, threads: int | None = None):
         self.path = path
-        self.fh = open(path, "w")
+        # File::create (dump_traj.rs:19-21): created or truncated
+        self.fd = os.open(path, os.O_WRONLY | os.O_CREAT | os.O_TRUNC, 0o644)
+        self.threads = threads if threads is not None else min(16, os.cpu_count() or 1)
 
     def write_step(self, positions, type_ids, box: SimulationBox, step: int) -> None:
-        h = box.h
-        out = [
-            "ITEM: TIMESTEP", str(step),
-            "ITEM: NUMBER OF ATOMS", str(len(type_ids)),
-            "ITEM: BOX BOUNDS pp pp pp",
-            f"0 {rust_float(h[0])}", f"0 {rust_float(h[4])}", f"0 {rust_float(h[8])}",
-            "ITEM: ATOMS id type x y z",
-        ]
-        for i, (t, p) in enumerate(zip(type_ids, positions)):
-            out.append(f"{i + 1} {int(t)} {rust_float(p[0])} {rust_float(p[1])} {rust_float(p[2])}")
-        self.fh.write("\n".join(out) + "\n")
+        import ctypes as C
+        from ._capi import check, lib
+
+        typ = np.ascontiguousarray(type_ids, dtype=np.uint32)
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(len(typ), 3)
+        h = np.ascontiguousarray(box.h, dtype=np.float64)
+        dp = C.POINTER(C.c_double)
+        check(lib().pis_dump_write_block(self.fd, int(step), len(typ), pos.ctypes.data_as(dp),
+                                         typ.ctypes.data_as(C.POINTER(C.c_uint32)), h.ctypes.data_as(dp),
+                                         self.threads))
 
     def close(self):
-        self.fh.close()
+        if self.fd >= 0:
+            os.close(self.fd)
+            self.fd = -1
 
 
 def thermo_line(step: int, th) -> str:
@@ -515,7 +520,7 @@ def run(ctx: Context, precision: str = "fp64", skin: float = 0.3, print_fn=print
             out.append(th)
             done += k
             if dumper is not None and done % nd == 0:
-                pos, _, _ = eng.get_state(0)
+                pos = eng.get_positions(0)
                 box = spec.box
                 if ens == "npt":  # the barostat moved the box
                     box = SimulationBox(eng.npt_box(0)[0].T.reshape(9))

@@ -263,6 +263,90 @@ def test_dump_block_format(tmp_path):
         "1 1 0 2.705 0.00001", "2 2 13.525 0.5 27"]
 
 
+RUST_DISPLAY = [(0.0, "0"), (-0.0, "-0"), (52.6, "52.6"), (1.0, "1"), (1e-5, "0.00001"),
+                (1.5e20, "150000000000000000000"), (0.1 + 0.2, "0.30000000000000004"), (5.26, "5.26"),
+                (float("inf"), "inf"), (float("-inf"), "-inf"), (float("nan"), "NaN"), (-2.5e-7, "-0.00000025"),
+                (1e23, "100000000000000000000000"), (2.0 ** 53 + 2, "9007199254740994"),
+                (5e-324, "0." + "0" * 323 + "5")]
+
+
+def _native_f64(x):
+    import ctypes as C
+    from pis_amd._capi import lib
+
+    buf = C.create_string_buffer(400)
+    n = C.c_size_t()
+    assert lib().pis_format_f64(x, buf, 400, C.byref(n)) == 0
+    assert n.value == len(buf.value)
+    return buf.value.decode()
+
+
+@pytest.mark.parametrize("x,text", RUST_DISPLAY)
+def test_native_float_display(x, text):
+    assert _native_f64(x) == text
+    assert S.rust_float(x) == text
+
+
+def test_native_float_display_agrees_with_python_on_random_doubles():
+    # two independent routes to Rust's `{}`: std::to_chars digits placed positionally in
+    # C++, and Python's shortest repr expanded through Decimal
+    rng = np.random.default_rng(5)
+    xs = np.concatenate([rng.uniform(-100, 100, 3000), rng.uniform(0, 1e-3, 1000),
+                         rng.integers(0, 2 ** 63, 4000, dtype=np.int64).view(np.float64)])
+    for x in xs:
+        assert _native_f64(float(x)) == S.rust_float(float(x)), repr(x)
+
+
+def _python_block(pos, typ, box, step):
+    h = box.h
+    lines = ["ITEM: TIMESTEP", str(step), "ITEM: NUMBER OF ATOMS", str(len(typ)), "ITEM: BOX BOUNDS pp pp pp",
+             f"0 {S.rust_float(h[0])}", f"0 {S.rust_float(h[4])}", f"0 {S.rust_float(h[8])}",
+             "ITEM: ATOMS id type x y z"]
+    lines += [f"{i + 1} {int(t)} {S.rust_float(p[0])} {S.rust_float(p[1])} {S.rust_float(p[2])}"
+              for i, (t, p) in enumerate(zip(typ, pos))]
+    return "\n".join(lines) + "\n"
+
+
+@pytest.mark.parametrize("threads", [1, 3, 16])
+def test_dump_blocks_match_python_text_for_any_thread_count(tmp_path, threads):
+    from pis_amd.box import SimulationBox
+
+    rng = np.random.default_rng(6)
+    n = 10000  # several 4096-atom formatting chunks
+    pos = rng.uniform(-1.0, 30.0, (n, 3))
+    pos[:5] = [[0.0, -0.0, 1e-9], [27.05, 1e5, 3.0], [0.1, 0.2, 0.3], [1.5e20, -7.0, 2.5e-7], [2.0, 4.0, 8.0]]
+    typ = rng.integers(1, 4, n).astype(np.uint32)
+    box = SimulationBox.cubic(30.25)
+    path = tmp_path / "t.lammpstrj"
+    d = S.DumpTraj(str(path), threads=threads)
+    d.write_step(pos, typ, box, 0)
+    d.write_step(pos[:7], typ[:7], box, 100)
+    d.close()
+    assert path.read_text() == _python_block(pos, typ, box, 0) + _python_block(pos[:7], typ[:7], box, 100)
+
+
+def test_dump_write_failure_is_an_io_error(tmp_path):
+    import ctypes as C
+    import os
+
+    from pis_amd._capi import lib
+
+    p = tmp_path / "ro.txt"
+    p.write_text("")
+    fd = os.open(str(p), os.O_RDONLY)
+    try:
+        pos = np.zeros(3)
+        typ = np.ones(1, np.uint32)
+        h = np.eye(3).reshape(9)
+        dp = C.POINTER(C.c_double)
+        st = lib().pis_dump_write_block(fd, 0, 1, pos.ctypes.data_as(dp), typ.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                        h.ctypes.data_as(dp), 1)
+    finally:
+        os.close(fd)
+    assert st == 13  # PIS_ERR_IO, DumpWriteError (errors.rs:50-55)
+    assert b"Failed to write trajectory" in lib().pis_last_error()
+
+
 def test_thermo_line_format():
     # 2.0005 is stored as 2.000500000000000167, so it rounds up, as Rust's {:.3} does
     assert S.thermo_line(12, [-1.23456, 2.0005, 0.7774999, 3.14159, -0.0004]) == "12 -1.235 2.001 0.777 3.142 -0.000"
