@@ -54,7 +54,12 @@ enum {
     NPT_SCALE = 19,     // velocity scaling matrix of the current step
     NPT_HINV_OLD = 28,  // h^-1 before the current step's box scaling
     NPT_RCL = 37,       // cell edge the grid needs (rcut + skin with margin)
-    NPT_COUNT = 40
+    NPT_HREF_INV = 40,  // h^-1 of the box the current neighbour list was built in
+    NPT_DEF = 49,       // deformation since that build, h h_ref^-1, of the current step
+    NPT_SKIN2 = 58,     // bound on |r - D r_ref|^2 that keeps the list complete this step
+    NPT_RCMAX = 59,     // largest cutoff
+    NPT_LIST_OK = 60,   // 1 once NPT_HREF_INV describes the current list
+    NPT_COUNT = 64
 };
 // per-block partial sums of the barostat kernels (SysDev::tens): v v^T, r F^T, (1/2) m v^2
 enum { TENS_K = 0, TENS_W = 9, TENS_KE = 18, TENS_COUNT = 20 };

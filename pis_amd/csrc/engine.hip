@@ -1034,11 +1034,12 @@ static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_
     }
     if (npt) {
         // verlet_step_npt_mtk (potential.rs:112-135): barostat and box update, the NVT step
-        // on the scaled state (its list rebuilt for the new box), the second scaling and
-        // momentum update; direct launches, step by step
+        // on the scaled state, the second scaling and momentum update; direct launches, step
+        // by step.  The tensor sums a step starts from are those the previous step ended
+        // with (nothing moves in between), so only a call's first step computes them.
         const int rec = out != nullptr ? 1 : 0;
         for (uint64_t t = 0; t < nsteps; ++t) {
-            launch_npt(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work, 0, rec);
+            if (t == 0) launch_npt(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work, 0, rec);
             launch_npt(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work, 1, rec);
             PIS_TRY(enqueue_one(c, t == 0, out != nullptr, true, false));
             launch_npt(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work, 2, rec);
@@ -1120,6 +1121,7 @@ static pis_status npt_init(pis_ctx* c, const pis_nhc_params* chain, const pis_mt
         for (int k = 0; k < 3; ++k) st[NPT_TARGET + 4 * k] = mtk->p_start;
         st[NPT_W] = ((double)(3 * h.plan.n)) * 0.0083144621 * chain->t_start * (mtk->tau * mtk->tau);
         st[NPT_RCL] = h.plan.rcl * (1.0 + 1e-12);
+        st[NPT_RCMAX] = h.plan.rcmax;  // NPT_LIST_OK = 0: the first step rebuilds
         PIS_HIP(hipMemcpyAsync(h.dev.npt, st, sizeof(st), hipMemcpyHostToDevice, c->stream));
     }
     PIS_HIP(hipStreamSynchronize(c->stream));

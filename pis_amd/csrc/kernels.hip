@@ -463,11 +463,22 @@ __global__ __launch_bounds__(kAtomThreads) void k_integrate_pos(const SysDev* __
     double z = gbl(st.z)[a] + (vz * dt + (gbl(st.fz)[a] / m) * 0.5 * dt2);
     // apply_boundary_conditions_pos (simulation_box.rs:29-42): s = h^-1 r; s -= floor(s); r = h s
     double dx, dy, dz;
+    double lim = s.half_skin2;
     if (s.tri) {
         wrap_tri(s.box, x, y, z);
-        dx = x - gbl(s.rx)[a];
-        dy = y - gbl(s.ry)[a];
-        dz = z - gbl(s.rz)[a];
+        double ux = gbl(s.rx)[a], uy = gbl(s.ry)[a], uz = gbl(s.rz)[a];
+        if (s.npt_on) {
+            // displacement from the build-time position carried along by the box
+            // deformation since the build (the periodic images move with the box)
+            const __attribute__((address_space(1))) double* npt = gbl(s.npt);
+            double D[9];
+            for (int k = 0; k < 9; ++k) D[k] = npt[NPT_DEF + k];
+            gemv3(D, ux, uy, uz, ux, uy, uz);
+            lim = npt[NPT_SKIN2];
+        }
+        dx = x - ux;
+        dy = y - uy;
+        dz = z - uz;
         mi_tri_fast(s.box, dx, dy, dz);
     } else {
         double sx = x * s.Linv[0], sy = y * s.Linv[1], sz = z * s.Linv[2];
@@ -485,7 +496,7 @@ __global__ __launch_bounds__(kAtomThreads) void k_integrate_pos(const SysDev* __
     gbl(st.y)[a] = y;
     gbl(st.z)[a] = z;
     const double d2 = dx * dx + dy * dy + dz * dz;
-    if (!(d2 <= s.half_skin2)) {  // also catches NaN
+    if (!(d2 <= lim)) {  // also catches NaN
         atomicOr(&s.ctl[CTL_REBUILD], 1);
         if (d2 != d2) atomicOr(&s.ctl[CTL_NAN], 1);
     }
@@ -1365,7 +1376,7 @@ __global__ __launch_bounds__(1024) void k_nhc_pre(const SysDev* __restrict__ sys
 {
     __shared__ double rtmp[64];
     const SysDev& s = sys[blockIdx.x];
-    __shared__ double ttmp[16 * TENS_COUNT];
+    __shared__ double ttmp[1024 + TENS_COUNT];
     if (!s.nhc_on || s.ctl[CTL_LEFT] == 0) return;
     double ke;
     if (s.npt_on) {
@@ -1496,26 +1507,27 @@ __device__ void mtk_scale(const double* npt, double dt, double factor, double* o
 }
 
 // Fixed-order sum of the per-block partials of one system (valid in thread 0).
-__device__ void sum_tens(const SysDev& s, double* v, double* tmp /* [16 waves * TENS_COUNT] */)
+__device__ void sum_tens(const SysDev& s, double* v, double* tmp /* [blockDim.x + TENS_COUNT] */)
 {
-    for (int k = 0; k < TENS_COUNT; ++k) v[k] = 0.0;
+    // the partials read as one flat array, coalesced: with a stride that is a multiple of
+    // TENS_COUNT every thread keeps one component; a fixed order of additions
+    const int stride = ((int)blockDim.x / TENS_COUNT) * TENS_COUNT;
+    const int t = threadIdx.x;
     const __attribute__((address_space(1))) double* T = gbl(static_cast<const double*>(s.tens));
-    for (int b = threadIdx.x; b < s.nblk; b += blockDim.x)
-        for (int k = 0; k < TENS_COUNT; ++k) v[k] += T[(size_t)b * TENS_COUNT + k];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    for (int k = 0; k < TENS_COUNT; ++k) {
-        double x = v[k];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        if (lane == 0) tmp[wid * TENS_COUNT + k] = x;
+    const int total = s.nblk * TENS_COUNT;
+    double acc = 0.0;
+    if (t < stride)
+        for (int i = t; i < total; i += stride) acc += T[i];
+    tmp[t] = acc;
+    __syncthreads();
+    if (t < TENS_COUNT) {
+        double x = 0.0;
+        for (int r = t; r < stride; r += TENS_COUNT) x += tmp[r];
+        tmp[blockDim.x + t] = x;
     }
     __syncthreads();
-    if (threadIdx.x == 0)
-        for (int k = 0; k < TENS_COUNT; ++k) {
-            double t = 0.0;
-            for (int w = 0; w < nw; ++w) t += tmp[w * TENS_COUNT + k];
-            v[k] = t;
-        }
+    if (t == 0)
+        for (int k = 0; k < TENS_COUNT; ++k) v[k] = tmp[blockDim.x + k];
 }
 
 // Per 256-atom block of an NPT system: phase NPT_SUMS sums v v^T (kinetic_tensor, no
@@ -1582,12 +1594,13 @@ __global__ __launch_bounds__(kAtomThreads) void k_npt_atoms(const SysDev* __rest
 
 // Start of an NPT step (potential.rs:118-125), one workgroup per system: the momentum
 // update from the pressure tensor of the live state, the step's velocity scaling
-// exp(-dt/2 eta_dot), the box h <- exp(dt eta_dot) h with its inverse and volume; the list
-// is rebuilt for the new box (the reference builds it in every force call).  A box that
+// exp(-dt/2 eta_dot), the box h <- exp(dt eta_dot) h with its inverse and volume, and the
+// displacement bound under which the list built in an earlier box still holds every pair
+// within the cutoff (the reference finds pairs afresh in every force call).  A box that
 // became thinner than the planned cell grid raises CTL_BOX.
-__global__ __launch_bounds__(256) void k_npt_begin(const SysDev* __restrict__ sys)
+__global__ __launch_bounds__(1024) void k_npt_begin(const SysDev* __restrict__ sys)
 {
-    __shared__ double tmp[4 * TENS_COUNT];
+    __shared__ double tmp[1024 + TENS_COUNT];
     const SysDev& s = sys[blockIdx.x];
     if (!s.npt_on || s.ctl[CTL_LEFT] == 0) return;
     double v[TENS_COUNT];
@@ -1608,7 +1621,26 @@ __global__ __launch_bounds__(256) void k_npt_begin(const SysDev* __restrict__ sy
         box[BOX_HINV + k] = hi[k];
     }
     box[BOX_VOL] = vol;
-    s.ctl[CTL_REBUILD] = 1;
+    // Keep the list while it stays complete.  With D = h h_ref^-1 and d_i = r_i - D r_i,ref
+    // (minimum image), a pair vector is r_ij = D r_ij,ref + d_j - d_i, so |r_ij| <= rc implies
+    // |r_ij,ref| <= (rc + 2 max|d|) / sigma_min(D).  sigma_min(D) >= 1 - |D - I|_F, and the
+    // list holds every pair within rc + skin at the build: the list is complete while
+    // max|d| <= ((1 - |D - I|_F)(rc + skin) - rc) / 2.
+    double def[9], dev = 0.0;
+    mat3_mul(h, npt + NPT_HREF_INV, def);
+    for (int k = 0; k < 9; ++k) {
+        const double e = def[k] - ((k & 3) == 0 ? 1.0 : 0.0);
+        dev += e * e;
+    }
+    const double rc = npt[NPT_RCMAX], skin = 2.0 * sqrt(s.half_skin2);
+    const double lim = 0.5 * ((1.0 - sqrt(dev)) * (rc + skin) - rc) * (1.0 - 1e-9);
+    for (int k = 0; k < 9; ++k) npt[NPT_DEF + k] = def[k];
+    if (npt[NPT_LIST_OK] != 1.0 || !(lim > 0.0)) {
+        s.ctl[CTL_REBUILD] = 1;
+        npt[NPT_SKIN2] = 0.0;
+    } else {
+        npt[NPT_SKIN2] = fmin(lim * lim, s.half_skin2);
+    }
     if (s.mode != MODE_WHOLE) {
         for (int k = 0; k < 3; ++k) {
             const double* p = h + 3 * ((k + 1) % 3);
@@ -1623,9 +1655,9 @@ __global__ __launch_bounds__(256) void k_npt_begin(const SysDev* __restrict__ sy
 // second momentum update; the step's thermo record gets the KE, temperature and pressure
 // of the final velocities and the NPT Hamiltonian (simulation.rs:106-114): PE + KE + chain
 // KE + chain PE (0: eta never moves) + barostat KE (npt.rs:60-62) + barostat PE (npt.rs:64-66).
-__global__ __launch_bounds__(256) void k_npt_end(const SysDev* __restrict__ sys, int record)
+__global__ __launch_bounds__(1024) void k_npt_end(const SysDev* __restrict__ sys, int record)
 {
-    __shared__ double tmp[4 * TENS_COUNT];
+    __shared__ double tmp[1024 + TENS_COUNT];
     const SysDev& s = sys[blockIdx.x];
     if (!s.npt_on || s.ctl[CTL_LEFT] == 0) return;
     double v[TENS_COUNT];
@@ -1657,6 +1689,16 @@ __global__ __launch_bounds__(256) void k_npt_end(const SysDev* __restrict__ sys,
     T[3] = (2.0 * ke) / ((double)(3 * (int64_t)s.n) * s.kb);
     T[4] = (2.0 * ke + trw) / (3.0 * vol);
     T[5] = trw;
+}
+
+// A list was built in the current box: the barostat's list check measures deformation
+// from here.
+__device__ void note_list_box(const SysDev& s)
+{
+    __attribute__((address_space(1))) double* npt = gbl(s.npt);
+    const __attribute__((address_space(1))) double* box = gbl(s.box);
+    for (int k = 0; k < 9; ++k) npt[NPT_HREF_INV + k] = box[BOX_HINV + k];
+    npt[NPT_LIST_OK] = 1.0;
 }
 
 __global__ __launch_bounds__(1024) void k_reduce(const SysDev* __restrict__ sys, int flip, int record,
@@ -1700,6 +1742,7 @@ __global__ __launch_bounds__(1024) void k_reduce(const SysDev* __restrict__ sys,
         ctl[CTL_STEP] = c_step + 1;
     }
     if (!isfinite(pe) || !isfinite(ke)) ctl[CTL_NAN] = 1;
+    if (c_rb && (step || flip)) note_list_box(s);
     if (step) {
         // the step wrote into the set it did not read: that set becomes live; the skin
         // check of positions advanced ahead decides the next step's rebuild
@@ -1745,6 +1788,7 @@ __global__ void k_flip(const SysDev* __restrict__ sys, int nsys)
     if (k >= nsys) return;
     const SysDev& s = sys[k];
     if (s.ctl[CTL_REBUILD]) {
+        note_list_box(s);
         s.ctl[CTL_CUR] ^= 1;
         s.ctl[CTL_REBUILD] = 0;
         s.ctl[CTL_NREBUILD] += 1;
@@ -1938,11 +1982,11 @@ void launch_npt(hipStream_t st, const SysDev* d_sys, int nsys, const WorkItem* a
     if (what == 0) {
         hipLaunchKernelGGL(k_npt_atoms, dim3(n_atom_work), dim3(kAtomThreads), 0, st, d_sys, atom_work, (int)NPT_SUMS);
     } else if (what == 1) {
-        hipLaunchKernelGGL(k_npt_begin, dim3(nsys), dim3(256), 0, st, d_sys);
+        hipLaunchKernelGGL(k_npt_begin, dim3(nsys), dim3(1024), 0, st, d_sys);
         hipLaunchKernelGGL(k_npt_atoms, dim3(n_atom_work), dim3(kAtomThreads), 0, st, d_sys, atom_work, (int)NPT_BEGIN);
     } else {
         hipLaunchKernelGGL(k_npt_atoms, dim3(n_atom_work), dim3(kAtomThreads), 0, st, d_sys, atom_work, (int)NPT_END);
-        hipLaunchKernelGGL(k_npt_end, dim3(nsys), dim3(256), 0, st, d_sys, record);
+        hipLaunchKernelGGL(k_npt_end, dim3(nsys), dim3(1024), 0, st, d_sys, record);
     }
 }
 

@@ -6,6 +6,9 @@
 * verlet_step_npt_mtk (potential.rs:112-135) run by Simulation::run's NPT loop tracks the
   oracle's restatement: thermo (with the barostat's energy terms in H) and the box matrix,
   which becomes general after the first step;
+* the neighbour list outlives box changes while the deformation-aware displacement check
+  allows it: after a 300-step NPT run with few rebuilds, the forces equal the oracle's
+  brute force over all pairs in the final box;
 * a barostat that squeezes the box below the planned cell grid stops with
   PIS_ERR_CAPACITY instead of missing pairs.
 
@@ -111,6 +114,38 @@ def test_npt_tracks_oracle(name, precision):
     # the box became general: off-diagonal terms from the symmetrised pressure tensor
     assert np.abs(h - np.diag(np.diag(h))).max() > 0.0
     assert np.allclose(mom, mom.T, rtol=0, atol=1e-12 * max(1.0, np.abs(mom).max()))
+
+
+@pytest.mark.parametrize("name", sorted(NPT_CASES))
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_npt_list_reuse_keeps_every_pair(name, precision):
+    make, t0, tau_t, p0, tau_p, _ = NPT_CASES[name]
+    spec = make()
+    steps = 300
+    print(f"{name} {precision}: N={spec.n}, {steps} NPT steps", flush=True)
+    with Engine(spec, precision) as eng:
+        eng.load()
+        eng.compute_potential()
+        eng.npt_init(t0, t0, tau_t, p0, tau_p, 3)
+        r0 = eng.stats()["rebuilds"]
+        eng.run_npt(spec.dt, steps)
+        rebuilds = eng.stats()["rebuilds"] - r0
+        pos, _, f = eng.get_state(0)
+        h, _ = eng.npt_box(0)
+    h0 = np.array(spec.box.h).reshape(3, 3).T
+    strain = np.abs(h @ np.linalg.inv(h0) - np.eye(3)).max()
+    print(f"rebuilds {rebuilds}, strain {strain:.3g}", flush=True)
+    assert strain > 1e-3  # the box did move
+    assert rebuilds <= steps // 4  # the list was kept across box changes
+    # the final state's forces against every pair of the final box
+    spec.box = SimulationBox(h.T.reshape(9))
+    spec.positions = pos
+    o = po.OSystem.from_spec(spec)
+    o.compute(po.MGR_BRUTE)
+    if precision == "fp64":
+        assert force_err(f, o.force) <= 1e-10
+    else:
+        assert force_rms_err(f, o.force) <= 1e-5
 
 
 def test_npt_squeezed_below_the_cell_grid_stops_with_capacity_error():
