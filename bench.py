@@ -148,9 +148,9 @@ def run_per_call(spec, steps):
 
 def run_thermostats(spec, nvt_steps, npt_steps):
     """NVT (Nose-Hoover chain, graph replay) and NPT (MTK barostat around the NVT step; the
-    box becomes general and the list is rebuilt every step, as the reference's force call
-    does) on the same system, mixed precision.  NPT's target is the starting pressure with
-    a heavy barostat (tau_p = 50), so the box stays inside its cell grid."""
+    box becomes general, and the list is kept while the deformation-aware displacement
+    check allows) on the same system, mixed precision.  NPT's target is the starting
+    pressure with a heavy barostat (tau_p = 50), so the box stays inside its cell grid."""
     from pis_amd import Engine
 
     out = {}
@@ -180,6 +180,44 @@ def run_thermostats(spec, nvt_steps, npt_steps):
         out["npt"] = {"value": round(spec.n * npt_steps / sec, 1), "unit": "atom-steps/s", "steps": npt_steps,
                       "ms_per_step": round(1e3 * sec / npt_steps, 4), "p_target": round(p0, 6), "tau_p": 50.0,
                       "max_tilt": float(np.abs(h - np.diag(np.diag(h))).max())}
+    return out
+
+
+def run_dump(spec, steps, cadence):
+    """Simulation::run's output path (simulation.rs:76-86) on the same system, mixed
+    precision: NVE in chunks of `cadence` steps with a thermo record per step and a
+    trajectory block (dump_traj.rs:33-74) after each chunk, written to /dev/null so the
+    figure is transfer + formatting, not the disk.  Positions leave the device while the
+    next chunk runs (pinned double buffer) and are formatted by the native writer on a
+    helper thread; the same loop without frames is timed beside it."""
+    from pis_amd import Engine
+    from pis_amd.script import AsyncFrames, DumpTraj
+
+    chunks = max(1, steps // cadence)
+    out = {"cadence": cadence, "frames": chunks, "steps": chunks * cadence,
+           "writer_threads": min(16, os.cpu_count() or 1)}
+    with Engine(spec, "mixed") as eng:
+        eng.load()
+        eng.compute_potential()
+        eng.run_nve(spec.dt, cadence)
+        t0 = time.perf_counter()
+        for _ in range(chunks):
+            eng.run_nve(spec.dt, cadence)
+        plain = time.perf_counter() - t0
+        dumper = DumpTraj(os.devnull)
+        frames = AsyncFrames(eng, dumper, spec.type_ids)
+        t0 = time.perf_counter()
+        for c in range(chunks):
+            eng.run_nve(spec.dt, cadence)
+            frames.frame((c + 1) * cadence, spec.box)
+        frames.close()
+        sec = time.perf_counter() - t0
+        dumper.close()
+    n = spec.n * chunks * cadence
+    out["value"] = round(n / sec, 1)
+    out["unit"] = "atom-steps/s"
+    out["without_frames"] = round(n / plain, 1)
+    out["s_per_frame"] = round(sec / chunks, 4)
     return out
 
 
@@ -339,6 +377,9 @@ def main():
                     help="steps of the PCIe-inclusive per-call path (0: skip)")
     ap.add_argument("--nvt-steps", type=int, default=2000, help="steps of the NVT leg (0: skip)")
     ap.add_argument("--npt-steps", type=int, default=300, help="steps of the NPT leg (0: skip)")
+    ap.add_argument("--dump-steps", type=int, default=200,
+                    help="steps of the trajectory-output leg, one frame per --dump-cadence steps (0: skip)")
+    ap.add_argument("--dump-cadence", type=int, default=20)
     ap.add_argument("--ensemble-steps", type=int, default=5000,
                     help="steps of the 512-system batched ensemble (0: skip)")
     ap.add_argument("--serve-jobs", type=int, default=100,
@@ -379,6 +420,12 @@ def main():
         thermostats = run_thermostats(spec, args.nvt_steps, args.npt_steps)
         for k, v in thermostats.items():
             log(f"{k}: {v['value']:.4e} atom-steps/s, {v['ms_per_step']} ms/step")
+    dump = None
+    if args.dump_steps:
+        log(f"trajectory output: {args.dump_steps} steps, a frame every {args.dump_cadence} (mixed)")
+        dump = run_dump(spec, args.dump_steps, args.dump_cadence)
+        log(f"dump: {dump['value']:.4e} atom-steps/s with frames ({dump['without_frames']:.4e} without), "
+            f"{dump['s_per_frame']} s per frame")
 
     ensemble = None
     if args.ensemble_steps:
@@ -485,6 +532,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "thermostat_barostat": thermostats,
+        "trajectory_output": dump,
         "batched_ensemble": ensemble,
         "pcie_inclusive_per_call": per_call,
         "job_latency": serving,

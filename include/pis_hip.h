@@ -284,6 +284,21 @@ pis_status pis_server_get_stats(pis_server* srv, pis_server_stats* out);
 /* Stops the worker; queued and running jobs end with an error. */
 pis_status pis_server_destroy(pis_server* srv);
 
+/* ---- Trajectory frames with asynchronous D2H ----------------------------------------
+ *
+ * Simulation::run writes a frame every dump_step steps (simulation.rs:76-78, default 1,
+ * simulation_context.rs:91).  pis_snapshot_positions enqueues, behind the steps already
+ * queued on the context, a copy of system sys's positions (original order, AoS xyz) and
+ * their transfer into host_pos on a copy stream of the context's own; it returns at once,
+ * so the next steps run while the frame travels.  host_pos (3 n doubles) must stay valid
+ * until pis_snapshot_wait returns; it is asynchronous only for pinned memory
+ * (pis_host_alloc).  One snapshot is in flight per context: a second call first waits for
+ * the previous one.  No reference counterpart for the two allocation calls. */
+pis_status pis_snapshot_positions(pis_ctx* ctx, uint32_t sys, double* host_pos);
+pis_status pis_snapshot_wait(pis_ctx* ctx);
+pis_status pis_host_alloc(size_t bytes, void** out);
+pis_status pis_host_free(void* p);
+
 /* ---- Trajectory writer (host only) ------------------------------------------------
  *
  * DumpTraj::write_step (src/writers/dump_traj.rs:33-74): appends one LAMMPS text block to

@@ -198,6 +198,10 @@ SIGNATURES = {
     "pis_dump_write_block": (C.c_int, [C.c_int, C.c_uint64, C.c_uint64, _DP, C.POINTER(C.c_uint32), _DP,
                                        C.c_int32]),
     "pis_format_f64": (C.c_int, [C.c_double, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "pis_snapshot_positions": (C.c_int, [_P, C.c_uint32, _DP]),
+    "pis_snapshot_wait": (C.c_int, [_P]),
+    "pis_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
+    "pis_host_free": (C.c_int, [C.c_void_p]),
 }
 
 _lib = None

@@ -102,6 +102,13 @@ struct pis_ctx {
     int nlanes = 1;
     void* d_xfer = nullptr;
     size_t xfer_bytes = 0;
+    // trajectory snapshots: positions copied on `stream`, sent to the host on snap_stream
+    // (default priority) while later steps run; one in flight at a time
+    hipStream_t snap_stream = nullptr;
+    hipEvent_t snap_ready = nullptr, snap_done = nullptr;
+    double* d_snap = nullptr;
+    size_t snap_bytes = 0;
+    bool snap_pending = false;
     // statistics / timing
     uint64_t steps = 0, force_launches = 0;
     bool timing = false;

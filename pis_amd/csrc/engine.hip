@@ -516,6 +516,13 @@ void pis::free_ctx(pis_ctx* c)
     if (c->d_atom_work) (void)hipFree(c->d_atom_work);
     if (c->d_tile_work) (void)hipFree(c->d_tile_work);
     if (c->d_xfer) (void)hipFree(c->d_xfer);
+    if (c->snap_stream) {
+        (void)hipStreamSynchronize(c->snap_stream);
+        (void)hipStreamDestroy(c->snap_stream);
+    }
+    if (c->snap_ready) (void)hipEventDestroy(c->snap_ready);
+    if (c->snap_done) (void)hipEventDestroy(c->snap_done);
+    if (c->d_snap) (void)hipFree(c->d_snap);
     for (auto e : c->ev) (void)hipEventDestroy(e);
     for (auto g : c->graph)
         if (g) (void)hipGraphExecDestroy(g);
@@ -1228,6 +1235,59 @@ pis_status pis_sys_set_state(pis_ctx* ctx, uint32_t sys, const double* pos, cons
 pis_status pis_sys_get_state(pis_ctx* ctx, uint32_t sys, double* pos, double* vel, double* force)
 {
     return sys_get_state(ctx, sys, pos, vel, force);
+}
+
+pis_status pis_host_alloc(size_t bytes, void** out)
+{
+    if (!out) return fail(PIS_ERR_INVALID_ARGUMENT, "null output");
+    *out = nullptr;
+    PIS_HIP(hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault));
+    return PIS_OK;
+}
+
+pis_status pis_host_free(void* p)
+{
+    if (p) PIS_HIP(hipHostFree(p));
+    return PIS_OK;
+}
+
+pis_status pis_snapshot_wait(pis_ctx* ctx)
+{
+    if (!ctx) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
+    if (!ctx->snap_pending) return PIS_OK;
+    ctx->snap_pending = false;
+    PIS_HIP(hipEventSynchronize(ctx->snap_done));
+    return PIS_OK;
+}
+
+pis_status pis_snapshot_positions(pis_ctx* ctx, uint32_t sys, double* host_pos)
+{
+    if (!ctx) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
+    if (sys >= ctx->sys.size()) return fail(PIS_ERR_INVALID_ARGUMENT, "system index out of range");
+    if (!host_pos) return fail(PIS_ERR_INVALID_ARGUMENT, "null destination");
+    PIS_TRY(pis_snapshot_wait(ctx));  // the device buffer is free again
+    if (!ctx->snap_stream) {
+        PIS_HIP(hipStreamCreateWithFlags(&ctx->snap_stream, hipStreamNonBlocking));
+        PIS_HIP(hipEventCreateWithFlags(&ctx->snap_ready, hipEventDisableTiming));
+        PIS_HIP(hipEventCreateWithFlags(&ctx->snap_done, hipEventDisableTiming));
+    }
+    const size_t n = (size_t)ctx->sys[sys].plan.n, bytes = 3 * n * sizeof(double);
+    if (ctx->snap_bytes < bytes) {
+        if (ctx->d_snap) PIS_HIP(hipFree(ctx->d_snap));
+        ctx->d_snap = nullptr;
+        ctx->snap_bytes = 0;
+        PIS_HIP(hipMalloc(&ctx->d_snap, bytes));
+        ctx->snap_bytes = bytes;
+    }
+    // behind every step already queued on the context's stream
+    launch_store_aos(ctx->stream, ctx->d_sys, (int)sys, (int)n, ctx->d_snap, nullptr, nullptr);
+    PIS_HIP(hipGetLastError());
+    PIS_HIP(hipEventRecord(ctx->snap_ready, ctx->stream));
+    PIS_HIP(hipStreamWaitEvent(ctx->snap_stream, ctx->snap_ready, 0));
+    PIS_HIP(hipMemcpyAsync(host_pos, ctx->d_snap, bytes, hipMemcpyDeviceToHost, ctx->snap_stream));
+    PIS_HIP(hipEventRecord(ctx->snap_done, ctx->snap_stream));
+    ctx->snap_pending = true;
+    return PIS_OK;
 }
 
 pis_status pis_compute_potential(pis_ctx* ctx, double* forces_inout, double* pe)

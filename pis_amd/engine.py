@@ -77,6 +77,29 @@ def thermo_to_array(recs) -> np.ndarray:
     return np.ctypeslib.as_array(recs).view(np.float64).reshape(-1, 7).copy()
 
 
+class PinnedArray:
+    """float64 array in pinned host memory (pis_host_alloc), the destination that makes a
+    position snapshot's transfer asynchronous.  Freed by free() or when collected."""
+
+    def __init__(self, shape):
+        count = int(np.prod(shape))
+        self._p = C.c_void_p()
+        check(lib().pis_host_alloc(8 * count, C.byref(self._p)))
+        self.array = np.ctypeslib.as_array((C.c_double * count).from_address(self._p.value)).reshape(shape)
+
+    def free(self):
+        if self._p:
+            self.array = None
+            lib().pis_host_free(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Engine:
     """Device-resident state of one or many systems (pis_ctx)."""
 
@@ -139,6 +162,16 @@ class Engine:
         pos = np.empty((self.specs[k].n, 3))
         check(lib().pis_sys_get_state(self._ptr, k, _dptr(pos), None, None))
         return pos
+
+    def snapshot_positions(self, k: int, dest: PinnedArray):
+        """Start copying system k's positions into dest behind the queued steps; returns at
+        once.  dest is valid after snapshot_wait()."""
+        if dest.array is None or dest.array.size != 3 * self.specs[k].n:
+            raise ValueError("snapshot destination has the wrong size")
+        check(lib().pis_snapshot_positions(self._ptr, k, _dptr(dest.array)))
+
+    def snapshot_wait(self):
+        check(lib().pis_snapshot_wait(self._ptr))
 
     def compute_potential(self) -> np.ndarray:
         pe = np.empty(self.nsys)

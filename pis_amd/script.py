@@ -474,6 +474,49 @@ class DumpTraj:
             self.fd = -1
 
 
+class AsyncFrames:
+    """Trajectory frames of a running engine with asynchronous D2H: each frame's positions
+    are snapshotted into one of two pinned buffers and travel while the next steps run; a
+    frame is formatted (on a helper thread) once the next frame is requested or at close()."""
+
+    def __init__(self, eng, dumper: DumpTraj, type_ids, k: int = 0):
+        from .engine import PinnedArray
+
+        self.eng, self.dumper, self.type_ids, self.k = eng, dumper, type_ids, k
+        n = eng.specs[k].n
+        self.buf = [PinnedArray((n, 3)), PinnedArray((n, 3))]
+        self.slot = 0
+        self.pending = None  # (step, box, slot) whose transfer is in flight
+        self.writer = None
+
+    def _start_pending(self):
+        if self.pending is None:
+            return
+        step, box, slot = self.pending
+        self.pending = None
+        if self.writer is not None:  # the frame before used the other buffer
+            self.writer.join()
+        self.eng.snapshot_wait()
+        self.writer = threading.Thread(target=self.dumper.write_step,
+                                       args=(self.buf[slot].array, self.type_ids, box, step))
+        self.writer.start()
+
+    def frame(self, step: int, box: SimulationBox) -> None:
+        # the writer of two frames back, which read this frame's buffer, was joined here
+        self._start_pending()
+        self.eng.snapshot_positions(self.k, self.buf[self.slot])
+        self.pending = (step, box, self.slot)
+        self.slot ^= 1
+
+    def close(self) -> None:
+        self._start_pending()
+        if self.writer is not None:
+            self.writer.join()
+            self.writer = None
+        for b in self.buf:
+            b.free()
+
+
 def thermo_line(step: int, th) -> str:
     """`{} {:.3} {:.3} {:.3} {:.3} {:.3}` of step, PE, KE, H, T, P (simulation.rs:83-86)."""
     return f"{step} {th[0]:.3f} {th[1]:.3f} {th[2]:.3f} {th[3]:.3f} {th[4]:.3f}"
@@ -481,16 +524,16 @@ def thermo_line(step: int, th) -> str:
 
 def run(ctx: Context, precision: str = "fp64", skin: float = 0.3, print_fn=print, dump: bool = True):
     """Simulation::run on the GPU: initial compute, then the steps in chunks of the dump
-    cadence, a thermo line per step and a trajectory block every dump_step steps.  The
-    text of a block is written by a helper thread while the device runs the next chunk.
-    Returns the thermo records [steps, 7]."""
+    cadence, a thermo line per step and a trajectory block every dump_step steps.  A
+    block's positions reach the host while the device runs the next chunk, and its text is
+    written by a helper thread meanwhile.  Returns the thermo records [steps, 7]."""
     from .engine import Engine
 
     ens = ctx.ensemble
     spec = to_spec(ctx, skin)
     nd = max(1, ctx.dump_args.dump_step)
     dumper = DumpTraj(os.path.join(ctx.base_dir, ctx.dump_args.file_name)) if dump else None
-    writer = None
+    frames = None
     out = []
     with Engine(spec, precision) as eng:
         eng.set_state(0, spec.positions, spec.velocities)
@@ -520,16 +563,14 @@ def run(ctx: Context, precision: str = "fp64", skin: float = 0.3, print_fn=print
             out.append(th)
             done += k
             if dumper is not None and done % nd == 0:
-                pos = eng.get_positions(0)
                 box = spec.box
                 if ens == "npt":  # the barostat moved the box
                     box = SimulationBox(eng.npt_box(0)[0].T.reshape(9))
-                if writer is not None:
-                    writer.join()
-                writer = threading.Thread(target=dumper.write_step, args=(pos, spec.type_ids, box, done))
-                writer.start()
-    if writer is not None:
-        writer.join()
+                if frames is None:
+                    frames = AsyncFrames(eng, dumper, spec.type_ids)
+                frames.frame(done, box)
+        if frames is not None:
+            frames.close()
     if dumper is not None:
         dumper.close()
     return np.concatenate(out) if out else np.zeros((0, 7))

@@ -73,3 +73,38 @@ def test_script_run_matches_oracle(tmp_path, ensemble):
             L = spec.box.lengths
             d = (rows - frames[step] + L / 2) % L - L / 2
             assert np.abs(d).max() <= 1e-9
+
+
+def test_position_snapshots_are_taken_behind_queued_steps():
+    """pis_snapshot_positions: the frame holds the positions after the steps queued before
+    it, arrives while later steps run, a second call waits for the first, and a batched
+    context snapshots the system asked for."""
+    from pis_amd import Engine, workloads as W
+    from pis_amd.engine import PinnedArray
+
+    a = W.lj_liquid(8, 0.8442, 0.9, seed=71, rcut=2.5, skin=0.3, dt=0.004)
+    b = W.lj_liquid(6, 0.8442, 1.1, seed=72, rcut=2.5, skin=0.3, dt=0.004)
+    with Engine([a, b], "fp64") as eng:
+        eng.load()
+        eng.compute_potential()
+        eng.run_nve(a.dt, 10)
+        ref = eng.get_positions(0)
+        buf = [PinnedArray((a.n, 3)), PinnedArray((a.n, 3))]
+        eng.snapshot_positions(0, buf[0])
+        eng.run_nve(a.dt, 5)  # runs while the frame travels
+        eng.snapshot_wait()
+        assert np.array_equal(buf[0].array, ref)
+        later = eng.get_positions(0)
+        assert not np.array_equal(later, ref)
+        eng.snapshot_positions(0, buf[0])
+        eng.snapshot_positions(0, buf[1])
+        eng.snapshot_wait()
+        assert np.array_equal(buf[0].array, later) and np.array_equal(buf[1].array, later)
+        other = PinnedArray((b.n, 3))
+        eng.snapshot_positions(1, other)
+        eng.snapshot_wait()
+        assert np.array_equal(other.array, eng.get_positions(1))
+        with pytest.raises(ValueError):
+            eng.snapshot_positions(1, buf[0])  # wrong size
+        for p in (*buf, other):
+            p.free()
