@@ -41,6 +41,9 @@
 #ifndef PIS_BUILD_UNROLL
 #define PIS_BUILD_UNROLL 8
 #endif
+#ifndef PIS_BUILD_SHIFTIN
+#define PIS_BUILD_SHIFTIN 1
+#endif
 #ifndef PIS_SORT_GRID_CAP
 #define PIS_SORT_GRID_CAP 1024
 #endif
@@ -729,6 +732,17 @@ __device__ __forceinline__ bool exact_within(const SysDev& s, const State& st, i
     return (dx * dx + dy * dy) + dz * dz <= r2max;
 }
 
+// m = 2 m + (r2 <= b): the compare lands in an SGPR pair that the add takes as its carry-in,
+// two VALU instructions per candidate bit (VALU only; v_cmp_ge(b, r2) is false for NaN).
+__device__ __forceinline__ void shift_in_le(uint32_t& m, float r2, float b)
+{
+    uint64_t c;
+    asm("v_cmp_ge_f32_e64 %1, %3, %2\n\t"
+        "v_addc_co_u32_e64 %0, %1, %0, %0, %1"
+        : "+v"(m), "=&s"(c)
+        : "v"(r2), "v"(b));
+}
+
 // Maximum of v over the wave (all lanes active), uniform: DPP within each row of 16 lanes
 // (quad swaps, half-row and row mirrors), then the four row maxima through scalar
 // registers.  No LDS round trips, unlike a shuffle reduction.
@@ -886,9 +900,19 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
                             blo = tt[j] == 3 ? tlo[3] : blo;
                             bhi = tt[j] == 3 ? thi[3] : bhi;
                         }
-                        mi |= (r2 <= blo ? 1u : 0u) << (k - k0 + j);
-                        mm |= (r2 <= bhi ? 1u : 0u) << (k - k0 + j);
+                        if (PIS_BUILD_SHIFTIN) {  // candidate k0 + i ends at bit n - 1 - i
+                            shift_in_le(mi, r2, blo);
+                            shift_in_le(mm, r2, bhi);
+                        } else {
+                            mi |= (r2 <= blo ? 1u : 0u) << (k - k0 + j);
+                            mm |= (r2 <= bhi ? 1u : 0u) << (k - k0 + j);
+                        }
                     }
+                }
+                if (PIS_BUILD_SHIFTIN) {  // back to candidate k0 + i at bit i
+                    const int n = k1 - k0;
+                    mi = n > 0 ? __builtin_bitreverse32(mi) >> (32 - n) : 0u;
+                    mm = n > 0 ? __builtin_bitreverse32(mm) >> (32 - n) : 0u;
                 }
                 if (half == 0) {
                     mlo = mi;
