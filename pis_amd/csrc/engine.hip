@@ -136,10 +136,29 @@ void set_bands(Plan* p)
             qmax = std::max(qmax, p->mode == MODE_TILE_REL ? 0.5 * (p->tile[k] + 2) * p->cellw[k] : p->L[k]);
             cwmax = std::max(cwmax, p->cellw[k]);
         }
-        // a general box: positions up to qmax per component, and the FP32 image takes two
-        // 3-term products (twice the error)
-        if (p->tri) qmax = p->qmax;
-        const double e = (p->tri ? 12.0 : 6.0) * u32 * qmax;
+        // a general box with raw staging: positions up to qmax per component, and the FP32
+        // image takes two 3-term products (twice the error).  Tile-relative staging of a
+        // general box: a halo spans (tile + 2) / (2 nc) of each fractional axis around the
+        // tile's centre, atoms drift up to skin/2 out of their cells, and a barostat may
+        // grow the box by the factor qmax holds over the starting extent.
+        bool img32 = false;
+        if (p->tri && p->mode == MODE_TILE_REL) {
+            double ext = 0.0, half = 0.0;
+            for (int r = 0; r < 3; ++r) {
+                double q = 0.0, e = 0.0;
+                for (int k = 0; k < 3; ++k) {
+                    q += std::fabs(p->h[r + 3 * k]) * (p->tile[k] + 2) / (2.0 * p->nc[k]);
+                    e += std::fabs(p->h[r + 3 * k]);
+                }
+                half = std::max(half, q);
+                ext = std::max(ext, e);
+            }
+            qmax = half * std::max(1.0, p->qmax / ext) + p->skin;
+        } else if (p->tri) {
+            qmax = p->qmax;
+            img32 = true;
+        }
+        const double e = (img32 ? 12.0 : 6.0) * u32 * qmax;
         p->wslack = 2.0 * u32 * qmax + 1e-9 * cwmax;
         for (auto& q : p->pp) {
             if (q.r2_list < 0.0) {
@@ -304,7 +323,12 @@ pis_status make_plan_impl(const pis_system_desc* d, Plan* p)
             if (!(p->tile[k] * p->cellw[k] + p->skin + 2.0 * p->rcmax < p->L[k] * (1.0 - 1e-9)))
                 p->mode = MODE_RAW_MI;
         }
-        if (p->tri) p->mode = MODE_RAW_MI;  // raw positions, per-pair image of the general box
+        // a general box stages tile-relative positions when two cells separate a tile from
+        // its own periodic image on every axis: then the condition above holds for any box
+        // whose cells stay wider than rcut + skin, as a barostat's boxes must
+        if (p->tri)
+            for (int k = 0; k < 3; ++k)
+                if (p->nc[k] - p->tile[k] < 2) p->mode = MODE_RAW_MI;
         const double stage = p->density * cellvol * (p->tile[0] + 2) * (p->tile[1] + 2) * (p->tile[2] + 2);
         p->scap = scap_for(stage * 1.1 + 48);
         if (p->scap < 0) return fail(PIS_ERR_UNSUPPORTED, "tile staging exceeds LDS capacity");
@@ -1117,7 +1141,8 @@ static pis_status npt_init(pis_ctx* c, const pis_nhc_params* chain, const pis_mt
         if (!p.tri) regraph = true;
         p.tri = true;
         p.qmax = std::max(p.qmax, 1.25 * ext);
-        if (p.mode == MODE_TILE_REL) p.mode = MODE_RAW_MI;
+        for (int k = 0; k < 3; ++k)  // tile-relative staging of a general box (see make_plan)
+            if (p.mode == MODE_TILE_REL && p.nc[k] - p.tile[k] < 2) p.mode = MODE_RAW_MI;
         set_bands(&p);
         h.dev.tri = 1;
         h.dev.mode = p.mode;

@@ -258,6 +258,38 @@ struct TileGeom {
     int x0, y0, z0, sx, sy, sz, ne, c0, c1;
 };
 
+// Cartesian centre of a tile: the cell-index centre times the cell widths (orthorhombic),
+// or the box applied to the fractional centre (general boxes, whose cells are slices of
+// the fractional coordinates).
+__device__ __forceinline__ void tile_center(const SysDev& s, const TileGeom& g, double& cx, double& cy,
+                                            double& cz)
+{
+    if (s.tri) {
+        gemv3(s.box + BOX_H, (g.x0 + 0.5 * g.sx) / s.nc[0], (g.y0 + 0.5 * g.sy) / s.nc[1],
+              (g.z0 + 0.5 * g.sz) / s.nc[2], cx, cy, cz);
+    } else {
+        cx = (g.x0 + 0.5 * g.sx) * s.cellw[0];
+        cy = (g.y0 + 0.5 * g.sy) * s.cellw[1];
+        cz = (g.z0 + 0.5 * g.sz) * s.cellw[2];
+    }
+}
+
+// Position relative to a tile centre at the image nearest to it (tile-relative staging).
+__device__ __forceinline__ void rel_to_center(const SysDev& s, double cx, double cy, double cz, double& x,
+                                              double& y, double& z)
+{
+    if (s.tri) {
+        x -= cx;
+        y -= cy;
+        z -= cz;
+        mi_tri_fast(s.box, x, y, z);
+    } else {
+        x = mi_fast(x - cx, s.L[0], s.Linv[0]);
+        y = mi_fast(y - cy, s.L[1], s.Linv[1]);
+        z = mi_fast(z - cz, s.L[2], s.Linv[2]);
+    }
+}
+
 __device__ __forceinline__ TileGeom tile_geom(const SysDev& s, int t)
 {
     TileGeom g;
@@ -326,11 +358,7 @@ __device__ int stage_tile(const SysDev& s, const State& st, const TileGeom& g, d
             reinterpret_cast<int*>(Px)[sl] = a;
         } else {
             double x = gbl(st.x)[a], y = gbl(st.y)[a], z = gbl(st.z)[a];
-            if (WHAT == STAGE_REL) {
-                x = mi_fast(x - cx, s.L[0], s.Linv[0]);
-                y = mi_fast(y - cy, s.L[1], s.Linv[1]);
-                z = mi_fast(z - cz, s.L[2], s.Linv[2]);
-            }
+            if (WHAT == STAGE_REL) rel_to_center(s, cx, cy, cz, x, y, z);
             Px[sl] = (T)x;
             Py[sl] = (T)y;
             Pz[sl] = (T)z;
@@ -787,11 +815,7 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
             own_slot(s, g, pref, a, &e0, &self, &ly, &lz);
     }
     double qx = xi, qy = yi, qz = zi;
-    if (!PAIRMI) {
-        qx = mi_fast(xi - cx, s.L[0], s.Linv[0]);
-        qy = mi_fast(yi - cy, s.L[1], s.Linv[1]);
-        qz = mi_fast(zi - cz, s.L[2], s.Linv[2]);
-    }
+    if (!PAIRMI) rel_to_center(s, cx, cy, cz, qx, qy, qz);
     const float qx32 = (float)qx, qy32 = (float)qy, qz32 = (float)qz;
     const float L32[3] = {(float)s.L[0], (float)s.L[1], (float)s.L[2]};
     const float Li32[3] = {(float)s.Linv[0], (float)s.Linv[1], (float)s.Linv[2]};
@@ -813,8 +837,8 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
         thi[t] = p.l_hi32;
         reach2 = fmax(reach2, p.l_hi);
     }
-    // Tile-relative staging keeps each 3-cell row sorted by x, so a row shrinks to the x
-    // window that can hold a partner: half-width sqrt(reach^2 - dy^2 - dz^2), with dy, dz
+    // Tile-relative staging of an orthorhombic box keeps each 3-cell row sorted by x, so a
+    // row shrinks to the x window that can hold a partner: half-width sqrt(reach^2 - dy^2 - dz^2), with dy, dz
     // the atom's distances to the row's cells (0 in its own y or z layer).  Slack of 1e-9
     // cell widths absorbs binning and rounding at the cell faces; the window widens by
     // twice the FP32 error of the staged x.
@@ -839,7 +863,7 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
                 const int e = e0 + oy * hx + oz * hxy;
                 rb = pref[e - 1];
                 re = pref[e + 2];
-                if (!PAIRMI) {
+                if (!PAIRMI && !TRI) {  // a tilted box's rows overlap in x: no window
                     const double gy = oy < 0 ? gym : (oy > 0 ? gyp : 0.0);
                     const double gz = oz < 0 ? gzm : (oz > 0 ? gzp : 0.0);
                     const double w2 = reach2 - gy * gy - gz * gz;
@@ -1012,9 +1036,8 @@ __global__ __launch_bounds__(kTileThreads) void k_build_list(const SysDev* __res
         const TileGeom g = tile_geom(s, w.idx);
         const int nt = s.ntypes;
         const bool rel = s.mode == MODE_TILE_REL;
-        const double cx = (g.x0 + 0.5 * g.sx) * s.cellw[0];
-        const double cy = (g.y0 + 0.5 * g.sy) * s.cellw[1];
-        const double cz = (g.z0 + 0.5 * g.sz) * s.cellw[2];
+        double cx, cy, cz;
+        tile_center(s, g, cx, cy, cz);
         for (int q = threadIdx.x; q < nt * nt; q += blockDim.x) ppl[q] = s.pp[q];
         int total;
         if (rel)
@@ -1035,7 +1058,10 @@ __global__ __launch_bounds__(kTileThreads) void k_build_list(const SysDev* __res
             const int a = base + lane;
             const bool active = a < a1;
             int n;
-            if (TRI)  // a general box always stages raw positions (per-pair image)
+            if (TRI && rel)  // tile-relative staging of a general box
+                n = nt == 1 ? build_atom<SCAP, false, false, true>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz)
+                            : build_atom<SCAP, false, true, true>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz);
+            else if (TRI)  // raw positions, per-pair image of the general box
                 n = nt == 1 ? build_atom<SCAP, true, false, true>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz)
                             : build_atom<SCAP, true, true, true>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz);
             else if (nt == 1)
@@ -1223,9 +1249,8 @@ __global__ __launch_bounds__(kTileThreads, TRI ? 1 : (MIXED ? PIS_FORCE_MIN_WAVE
                        __builtin_amdgcn_readfirstlane(s.ctl[CTL_LEFT]) != 1 && !s.nhc_on;
     const TileGeom g = tile_geom(s, w.idx);
     const bool rel = s.mode == MODE_TILE_REL;
-    const double cx = (g.x0 + 0.5 * g.sx) * s.cellw[0];
-    const double cy = (g.y0 + 0.5 * g.sy) * s.cellw[1];
-    const double cz = (g.z0 + 0.5 * g.sz) * s.cellw[2];
+    double cx, cy, cz;
+    tile_center(s, g, cx, cy, cz);
     const int nt = s.ntypes;
     for (int q = threadIdx.x; q < nt * nt; q += blockDim.x) ppl[q] = s.pp[q];
     int total;
@@ -1241,18 +1266,21 @@ __global__ __launch_bounds__(kTileThreads, TRI ? 1 : (MIXED ? PIS_FORCE_MIN_WAVE
             const double xi = gbl(st.x)[a], yi = gbl(st.y)[a], zi = gbl(st.z)[a];
             const int ti = gbl(st.type)[a];
             double qx = xi, qy = yi, qz = zi;
-            if (rel) {
-                qx = mi_fast(xi - cx, s.L[0], s.Linv[0]);
-                qy = mi_fast(yi - cy, s.L[1], s.Linv[1]);
-                qz = mi_fast(zi - cz, s.L[2], s.Linv[2]);
-            }
+            if (rel) rel_to_center(s, cx, cy, cz, qx, qy, qz);
             Acc acc = {0.0, 0.0, 0.0, 0.0, 0.0};
             const ExactCtx ex = {&s, &st, pref, ecell, g.ne, xi, yi, zi};
-            if (TRI) {  // a general box always stages raw positions (per-pair image)
-                if (nt == 1)
-                    atom_forces<SCAP, MIXED, false, true, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
-                else
-                    atom_forces<SCAP, MIXED, true, true, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+            if (TRI) {  // general box: tile-relative staging, or raw with a per-pair image
+                if (rel) {
+                    if (nt == 1)
+                        atom_forces<SCAP, MIXED, false, false, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+                    else
+                        atom_forces<SCAP, MIXED, true, false, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+                } else {
+                    if (nt == 1)
+                        atom_forces<SCAP, MIXED, false, true, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+                    else
+                        atom_forces<SCAP, MIXED, true, true, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+                }
             } else if (nt == 1) {
                 if (rel)
                     atom_forces<SCAP, MIXED, false, false, false>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
