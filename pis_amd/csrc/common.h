@@ -65,9 +65,12 @@ enum {
 enum { TENS_K = 0, TENS_W = 9, TENS_KE = 18, TENS_COUNT = 20 };
 
 enum { MODE_TILE_REL = 0, MODE_RAW_MI = 1, MODE_WHOLE = 2 };
-// kinds of systems a launch covers: the build and force kernels come in an orthorhombic and
-// a general-box instantiation, each taking only its own systems' tiles
-enum { KIND_ORTHO = 1, KIND_TRI = 2 };
+// kinds of systems a launch covers: the build kernel comes in an orthorhombic and a
+// general-box instantiation, the force kernel also splits general boxes into tile-relative
+// staging (KIND_TRI) and raw staging with a per-pair image (KIND_TRI_RAW: whole-box systems),
+// whose registers would otherwise lower the occupancy of the common case; each takes only
+// its own systems' tiles
+enum { KIND_ORTHO = 1, KIND_TRI = 2, KIND_TRI_RAW = 4 };
 
 struct PairParam {
     double r2_force;  // largest r^2 with sqrt_rn(r^2) <= rcut (-1: pair absent)

@@ -132,10 +132,14 @@ pis_status read_ctl(pis_ctx* c, uint32_t k, int32_t* ctl);
 pis_status write_ctl(pis_ctx* c, uint32_t k, const int32_t* ctl);
 pis_status enqueue_classes(pis_ctx* c, bool build, int force);
 // KIND_* bits of the context's systems (which build/force instantiations a launch needs)
+inline int plan_kind(const Plan& p)
+{
+    return !p.tri ? KIND_ORTHO : (p.mode == MODE_TILE_REL ? KIND_TRI : KIND_TRI_RAW);
+}
 inline int ctx_kinds(const pis_ctx* c)
 {
     int k = 0;
-    for (const auto& h : c->sys) k |= h.plan.tri ? KIND_TRI : KIND_ORTHO;
+    for (const auto& h : c->sys) k |= plan_kind(h.plan);
     return k;
 }
 pis_status enqueue_rebuild(pis_ctx* c);

@@ -1216,8 +1216,9 @@ __device__ __forceinline__ void atom_forces(const SysDev& s, const double* Px, c
     }
 }
 
-template <int SCAP, bool MIXED, bool TRI>
-__global__ __launch_bounds__(kTileThreads, TRI ? 1 : (MIXED ? PIS_FORCE_MIN_WAVES_MIXED : PIS_FORCE_MIN_WAVES_FP64)) void k_force(const SysDev* __restrict__ sys,
+// KIND: KIND_ORTHO, KIND_TRI (tile-relative general box) or KIND_TRI_RAW.
+template <int SCAP, bool MIXED, int KIND>
+__global__ __launch_bounds__(kTileThreads, KIND == KIND_TRI_RAW ? 1 : (MIXED ? PIS_FORCE_MIN_WAVES_MIXED : PIS_FORCE_MIN_WAVES_FP64)) void k_force(const SysDev* __restrict__ sys,
                                                          const WorkItem* __restrict__ tiles, int ntw,
                                                          int integrate)
 {
@@ -1231,7 +1232,9 @@ __global__ __launch_bounds__(kTileThreads, TRI ? 1 : (MIXED ? PIS_FORCE_MIN_WAVE
 
     const WorkItem w = tiles[xcd_remap(blockIdx.x, ntw)];
     const SysDev& s = sys[w.sys];
-    if ((s.tri != 0) != TRI) return;                 // the other instantiation takes this tile
+    constexpr bool TRI = KIND != KIND_ORTHO;
+    const int kind = !s.tri ? KIND_ORTHO : (s.mode == MODE_TILE_REL ? KIND_TRI : KIND_TRI_RAW);
+    if (kind != KIND) return;                        // another instantiation takes this tile
     if (integrate && s.ctl[CTL_LEFT] == 0) return;  // frozen system: the whole block leaves
     // read set: the live set, or the freshly sorted one on a rebuild step; a step writes
     // its results into the other set (wt), which becomes live at the end of the step
@@ -1248,7 +1251,7 @@ __global__ __launch_bounds__(kTileThreads, TRI ? 1 : (MIXED ? PIS_FORCE_MIN_WAVE
     const bool ahead = integrate && __builtin_amdgcn_readfirstlane(s.ctl[CTL_REMAIN]) > 1 &&
                        __builtin_amdgcn_readfirstlane(s.ctl[CTL_LEFT]) != 1 && !s.nhc_on;
     const TileGeom g = tile_geom(s, w.idx);
-    const bool rel = s.mode == MODE_TILE_REL;
+    const bool rel = KIND == KIND_TRI || (KIND == KIND_ORTHO && s.mode == MODE_TILE_REL);
     double cx, cy, cz;
     tile_center(s, g, cx, cy, cz);
     const int nt = s.ntypes;
@@ -1270,7 +1273,7 @@ __global__ __launch_bounds__(kTileThreads, TRI ? 1 : (MIXED ? PIS_FORCE_MIN_WAVE
             Acc acc = {0.0, 0.0, 0.0, 0.0, 0.0};
             const ExactCtx ex = {&s, &st, pref, ecell, g.ne, xi, yi, zi};
             if (TRI) {  // general box: tile-relative staging, or raw with a per-pair image
-                if (rel) {
+                if (KIND == KIND_TRI) {
                     if (nt == 1)
                         atom_forces<SCAP, MIXED, false, false, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
                     else
@@ -1421,6 +1424,7 @@ __device__ __forceinline__ void sum_partials(const SysDev& s, double* v, double*
 }
 
 __device__ void sum_tens(const SysDev& s, double* v, double* tmp);
+__device__ double sum_tens_ke(const SysDev& s, double* tmp);
 
 // Start of an NVT step (potential.rs:41-45): KE of the current velocities (the last
 // force launch's partials), chain forces and half step, and the velocity scale.
@@ -1434,9 +1438,7 @@ __global__ __launch_bounds__(1024) void k_nhc_pre(const SysDev* __restrict__ sys
     if (s.npt_on) {
         // NPT: the barostat rescaled the velocities since the last force launch; their KE
         // was summed per block right after the scaling
-        double u[TENS_COUNT];
-        sum_tens(s, u, ttmp);
-        ke = u[TENS_KE];
+        ke = sum_tens_ke(s, ttmp);
     } else {
         double v[4];
         sum_partials(s, v, rtmp);
@@ -1559,6 +1561,27 @@ __device__ void mtk_scale(const double* npt, double dt, double factor, double* o
 }
 
 // Fixed-order sum of the per-block partials of one system (valid in thread 0).
+// Sum of the KE column of the partials alone; result valid in thread 0.
+__device__ double sum_tens_ke(const SysDev& s, double* tmp /* [blockDim.x] */)
+{
+    const int t = threadIdx.x;
+    const __attribute__((address_space(1))) double* T = gbl(static_cast<const double*>(s.tens));
+    double a0 = 0.0, a1 = 0.0;
+    int b = t;
+    for (; b + (int)blockDim.x < s.nblk; b += 2 * (int)blockDim.x) {
+        a0 += T[(size_t)b * TENS_COUNT + TENS_KE];
+        a1 += T[(size_t)(b + blockDim.x) * TENS_COUNT + TENS_KE];
+    }
+    if (b < s.nblk) a0 += T[(size_t)b * TENS_COUNT + TENS_KE];
+    tmp[t] = a0 + a1;
+    __syncthreads();
+    for (int w = (int)blockDim.x >> 1; w > 0; w >>= 1) {
+        if (t < w) tmp[t] += tmp[t + w];
+        __syncthreads();
+    }
+    return tmp[0];
+}
+
 __device__ void sum_tens(const SysDev& s, double* v, double* tmp /* [blockDim.x + TENS_COUNT] */)
 {
     // the partials read as one flat array, coalesced: with a stride that is a multiple of
@@ -1567,10 +1590,16 @@ __device__ void sum_tens(const SysDev& s, double* v, double* tmp /* [blockDim.x 
     const int t = threadIdx.x;
     const __attribute__((address_space(1))) double* T = gbl(static_cast<const double*>(s.tens));
     const int total = s.nblk * TENS_COUNT;
-    double acc = 0.0;
-    if (t < stride)
-        for (int i = t; i < total; i += stride) acc += T[i];
-    tmp[t] = acc;
+    // eight independent loads in flight per thread (the loop is latency-bound otherwise)
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (t < stride) {
+        int i = t;
+        for (; i + 7 * stride < total; i += 8 * stride)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc[u] += T[i + u * stride];
+        for (int u = 0; i < total; i += stride, ++u) acc[u] += T[i];
+    }
+    tmp[t] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     __syncthreads();
     if (t < TENS_COUNT) {
         double x = 0.0;
@@ -1592,7 +1621,9 @@ enum { NPT_SUMS = 0, NPT_BEGIN = 1, NPT_END = 2 };
 __global__ __launch_bounds__(kAtomThreads) void k_npt_atoms(const SysDev* __restrict__ sys,
                                                              const WorkItem* __restrict__ work, int phase)
 {
-    __shared__ double tmp[4 * TENS_COUNT];
+    static_assert(kAtomThreads == 256, "the block sums below split 256 values into 8 x 32");
+    __shared__ double red[10 * (kAtomThreads + 1)];
+    __shared__ double red2[8 * 10];
     const WorkItem w = work[blockIdx.x];
     const SysDev& s = sys[w.sys];
     if (!s.npt_on || s.ctl[CTL_LEFT] == 0) return;
@@ -1629,18 +1660,39 @@ __global__ __launch_bounds__(kAtomThreads) void k_npt_atoms(const SysDev* __rest
         const double m = s.mass[gbl(st.type)[a]];
         v[TENS_KE] = 0.5 * m * ((vx * vx + vy * vy) + vz * vz);
     }
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int k = 0; k < TENS_COUNT; ++k) {
-        double x = v[k];
+    // Block sums in a fixed order.  NPT_BEGIN: the KE alone (the other columns are not
+    // read before NPT_END rewrites them).  Otherwise every component: its 256 values go
+    // through LDS, ten components at a time, and are added as 8 runs of 32, then the 8 runs.
+    const int t = threadIdx.x;
+    __attribute__((address_space(1))) double* out = gbl(s.tens) + (size_t)(w.idx / kAtomThreads) * TENS_COUNT;
+    if (phase == NPT_BEGIN) {
+        double x = v[TENS_KE];
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        if (lane == 0) tmp[wid * TENS_COUNT + k] = x;
+        if ((t & 63) == 0) red2[t >> 6] = x;
+        __syncthreads();
+        if (t == 0) out[TENS_KE] = (red2[0] + red2[1]) + (red2[2] + red2[3]);
+        return;
     }
-    __syncthreads();
-    if (threadIdx.x < TENS_COUNT) {
-        double t = 0.0;
-        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += tmp[q * TENS_COUNT + threadIdx.x];
-        gbl(s.tens)[(size_t)(w.idx / kAtomThreads) * TENS_COUNT + threadIdx.x] = t;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+#pragma unroll
+        for (int k = 0; k < 10; ++k) red[k * (kAtomThreads + 1) + t] = v[10 * r + k];
+        __syncthreads();
+        if (t < 80) {
+            const int k = t % 10, part = t / 10;
+            const double* q = red + k * (kAtomThreads + 1) + part * 32;
+            double x = 0.0;
+            for (int i = 0; i < 32; ++i) x += q[i];
+            red2[part * 10 + k] = x;
+        }
+        __syncthreads();
+        if (t < 10) {
+            double x = 0.0;
+            for (int part = 0; part < 8; ++part) x += red2[part * 10 + t];
+            out[10 * r + t] = x;
+        }
+        __syncthreads();  // red and red2 are rewritten by the next round
     }
 }
 
@@ -1969,7 +2021,7 @@ void launch_build_list(hipStream_t st, const SysDev* d_sys, const WorkItem* tile
 #define PIS_BUILD_TRI(S) \
     hipLaunchKernelGGL((k_build_list<S, true>), dim3(gt), dim3(kTileThreads), 0, st, d_sys, tile_work, n_tile_work)
     if (kinds & KIND_ORTHO) PIS_SCAP_DISPATCH(scap, PIS_BUILD);
-    if (kinds & KIND_TRI) PIS_SCAP_DISPATCH(scap, PIS_BUILD_TRI);
+    if (kinds & (KIND_TRI | KIND_TRI_RAW)) PIS_SCAP_DISPATCH(scap, PIS_BUILD_TRI);
 #undef PIS_BUILD
 #undef PIS_BUILD_TRI
 }
@@ -1979,14 +2031,14 @@ void launch_force(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work
 {
     if (n_tile_work <= 0) return;
     const dim3 g(n_tile_work), b(kTileThreads);
-#define PIS_FORCE_M(S) \
-    hipLaunchKernelGGL((k_force<S, true, false>), g, b, 0, st, d_sys, tile_work, n_tile_work, integrate)
-#define PIS_FORCE_D(S) \
-    hipLaunchKernelGGL((k_force<S, false, false>), g, b, 0, st, d_sys, tile_work, n_tile_work, integrate)
-#define PIS_FORCE_MT(S) \
-    hipLaunchKernelGGL((k_force<S, true, true>), g, b, 0, st, d_sys, tile_work, n_tile_work, integrate)
-#define PIS_FORCE_DT(S) \
-    hipLaunchKernelGGL((k_force<S, false, true>), g, b, 0, st, d_sys, tile_work, n_tile_work, integrate)
+#define PIS_FORCE_LAUNCH(S, M, K) \
+    hipLaunchKernelGGL((k_force<S, M, K>), g, b, 0, st, d_sys, tile_work, n_tile_work, integrate)
+#define PIS_FORCE_M(S) PIS_FORCE_LAUNCH(S, true, KIND_ORTHO)
+#define PIS_FORCE_D(S) PIS_FORCE_LAUNCH(S, false, KIND_ORTHO)
+#define PIS_FORCE_MT(S) PIS_FORCE_LAUNCH(S, true, KIND_TRI)
+#define PIS_FORCE_DT(S) PIS_FORCE_LAUNCH(S, false, KIND_TRI)
+#define PIS_FORCE_MR(S) PIS_FORCE_LAUNCH(S, true, KIND_TRI_RAW)
+#define PIS_FORCE_DR(S) PIS_FORCE_LAUNCH(S, false, KIND_TRI_RAW)
     if (kinds & KIND_ORTHO) {
         if (mixed)
             PIS_SCAP_DISPATCH(scap, PIS_FORCE_M);
@@ -1999,10 +2051,19 @@ void launch_force(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work
         else
             PIS_SCAP_DISPATCH(scap, PIS_FORCE_DT);
     }
+    if (kinds & KIND_TRI_RAW) {
+        if (mixed)
+            PIS_SCAP_DISPATCH(scap, PIS_FORCE_MR);
+        else
+            PIS_SCAP_DISPATCH(scap, PIS_FORCE_DR);
+    }
+#undef PIS_FORCE_LAUNCH
 #undef PIS_FORCE_M
 #undef PIS_FORCE_D
 #undef PIS_FORCE_MT
 #undef PIS_FORCE_DT
+#undef PIS_FORCE_MR
+#undef PIS_FORCE_DR
 }
 
 void launch_reduce(hipStream_t st, const SysDev* d_sys, int nsys, int flip, int record, int step)

@@ -106,7 +106,7 @@ pis_status admit(pis_server* sv, const std::shared_ptr<Job>& j)
     for (const auto& q : c->classes) {
         if (q.scap != p.scap) continue;
         launch_force(c->stream, c->d_sys, c->d_tile_work + q.first + (q.count - p.ntile), p.ntile, q.scap,
-                     c->mixed, 0, p.tri ? KIND_TRI : KIND_ORTHO);
+                     c->mixed, 0, plan_kind(p));
     }
     int32_t ctl[CTL_COUNT];
     PIS_TRY(read_ctl(c, k, ctl));
