@@ -10,10 +10,18 @@
 //   PotentialManager::compute_potential          src/potentials/potential.rs:12-13
 //   PotentialManager::verlet_step_nve (default)  src/potentials/potential.rs:15-33
 //   PairPotentialManager table API               src/potentials/potential.rs:148-193
-//   Simulation::run (NVE)                        src/simulation.rs:8-88
+//   PotentialManager::verlet_step_nvt_nhc/_npt_mtk (device-resident runs)
+//                                                src/potentials/potential.rs:35-58, 112-135
+//   NHChainArgs / MTKBarostatArgs                src/ensemble/nvt.rs:95-108, npt.rs:66-87
+//   DumpTraj::write_step                         src/writers/dump_traj.rs:33-74
+//   Simulation::run (NVE, NVT, NPT)              src/simulation.rs:8-116
 //   PisError                                     src/errors.rs:5-100
 #pragma once
 
+#include <fcntl.h>
+#include <unistd.h>
+
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -188,6 +196,32 @@ public:
     }
 };
 
+// Arguments of NHThermostatChain::new_from_args (nvt.rs:95-108) and of
+// MTKBarostat::new_from_args (npt.rs:66-87), as `fix nvt` / `fix npt` give them.
+struct NHChainArgs {
+    double start_temperature = 0.0, end_temperature = 0.0, tau = 0.0;
+    int32_t chain = 3;
+};
+struct MTKBarostatArgs {
+    double start_pressure = 0.0, tau = 0.0;
+};
+enum class Ensemble { NVE, NVT, NPT };
+
+// DumpArgs (simulation_context.rs:78-101): a frame every dump_step steps.
+struct DumpArgs {
+    size_t dump_step = 1;
+    std::string file_name = "dump.lammpstrj";
+};
+
+// f64 as Rust's `{}` prints it (shortest round-trip digits, positional).
+inline std::string rust_display(double x)
+{
+    char buf[400];
+    size_t n = 0;
+    check(pis_format_f64(x, buf, sizeof buf, &n));
+    return std::string(buf, n);
+}
+
 // GPU LJ manager: the pair table of PairPotentialManager and compute_potential /
 // verlet_step_nve through the C ABI.  A context is created on first use for the atoms'
 // shape and reused while the shape, box and types do not change.
@@ -252,6 +286,53 @@ public:
         check(pis_get_state(ctx_, atoms.positions.data(), atoms.velocities.data(), atoms.forces.data()));
     }
 
+    // NHThermostatChain::new_from_args on the device (xi = eta = G = 0, target = start).
+    void nvt_init(const Atoms& atoms, const NHChainArgs& c)
+    {
+        ensure(atoms);
+        const pis_nhc_params p{c.start_temperature, c.end_temperature, c.tau, c.chain, 0};
+        check(pis_nvt_init(ctx_, &p));
+    }
+
+    // The chain as above plus MTKBarostat::new_from_args (target p0 I, zero momentum).
+    void npt_init(const Atoms& atoms, const NHChainArgs& c, const MTKBarostatArgs& b)
+    {
+        ensure(atoms);
+        const pis_nhc_params p{c.start_temperature, c.end_temperature, c.tau, c.chain, 0};
+        const pis_mtk_params m{b.start_pressure, b.tau, {0.0, 0.0}};
+        check(pis_npt_init(ctx_, &p, &m));
+    }
+
+    // nsteps of verlet_step_nvt_nhc with Simulation::step's target ramp continued from
+    // step ramp_step0 of a ramp_total-step run (simulation.rs:53-57).
+    void run_nvt(Atoms& atoms, double dt, uint64_t nsteps, uint64_t ramp_step0, uint64_t ramp_total,
+                 std::vector<pis_thermo>* thermo = nullptr)
+    {
+        ensure(atoms);
+        check(pis_set_state(ctx_, atoms.positions.data(), atoms.velocities.data(), atoms.forces.data()));
+        if (thermo) thermo->resize(nsteps);
+        check(pis_batch_verlet_step_nvt_ramp(ctx_, &dt, nsteps, ramp_step0, ramp_total,
+                                             thermo ? thermo->data() : nullptr));
+        check(pis_get_state(ctx_, atoms.positions.data(), atoms.velocities.data(), atoms.forces.data()));
+    }
+
+    // nsteps of verlet_step_npt_mtk; the barostat moved the box on the device, so
+    // atoms.sim_box takes the new h (h_inv by SimulationBox::new, as scale_box does).
+    void run_npt(Atoms& atoms, double dt, uint64_t nsteps, uint64_t ramp_step0, uint64_t ramp_total,
+                 std::vector<pis_thermo>* thermo = nullptr)
+    {
+        ensure(atoms);
+        check(pis_set_state(ctx_, atoms.positions.data(), atoms.velocities.data(), atoms.forces.data()));
+        if (thermo) thermo->resize(nsteps);
+        check(pis_batch_verlet_step_npt(ctx_, &dt, nsteps, ramp_step0, ramp_total,
+                                        thermo ? thermo->data() : nullptr));
+        check(pis_get_state(ctx_, atoms.positions.data(), atoms.velocities.data(), atoms.forces.data()));
+        std::array<double, 9> h{};
+        check(pis_npt_get_box(ctx_, 0, h.data(), nullptr));
+        atoms.sim_box = SimulationBox::new_(h, atoms.sim_box.pbc);
+        h_ = h;  // the context follows the box: no new context for the next call
+    }
+
 private:
     void release()
     {
@@ -309,23 +390,75 @@ private:
     std::vector<double> masses_;
 };
 
-// Simulation::run for the NVE ensemble (simulation.rs:8-39, 66-88): one initial
-// compute_potential from zero forces, then `steps` device-resident steps, one thermo line
-// "step PE KE H T P" per step in the reference's {:.3} format.
+// DumpTraj (dump_traj.rs:12-75): File::create, then one LAMMPS text block per write_step,
+// formatted by the library's writer.
+class DumpTraj {
+public:
+    explicit DumpTraj(const DumpArgs& args) : path_(args.file_name)
+    {
+        fd_ = ::open(path_.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
+        if (fd_ < 0) throw PisError(PIS_ERR_IO, "Failed to create trajectory file '" + path_ + "'");
+    }
+    ~DumpTraj()
+    {
+        if (fd_ >= 0) ::close(fd_);
+    }
+    DumpTraj(const DumpTraj&) = delete;
+    DumpTraj& operator=(const DumpTraj&) = delete;
+
+    void write_step(const Atoms& atoms, uint64_t step)
+    {
+        check(pis_dump_write_block(fd_, step, atoms.n_atoms, atoms.positions.data(), atoms.type_ids.data(),
+                                   atoms.sim_box.h.data(), 0));
+    }
+
+private:
+    std::string path_;
+    int fd_ = -1;
+};
+
+// Simulation::run (simulation.rs:8-116): the step-0 frame, one compute_potential from zero
+// forces ("0 PE" printed as Rust's {}), then the steps on the device in chunks that end at
+// every dump_step, one thermo line "step PE KE H T P" ({:.3}) per step and a frame at the
+// end of each chunk.  H carries the chain's (NVT) and the barostat's (NPT) energies.
+// Without dump arguments no trajectory is written.
 struct Simulation {
     static std::vector<pis_thermo> run(LJGpuManager& mgr, Atoms& atoms, double dt, uint64_t steps,
-                                       FILE* out = nullptr)
+                                       FILE* out = nullptr, Ensemble ensemble = Ensemble::NVE,
+                                       const NHChainArgs& chain = {}, const MTKBarostatArgs& barostat = {},
+                                       const DumpArgs* dump = nullptr)
     {
+        std::unique_ptr<DumpTraj> dumper;
+        if (dump) {
+            if (dump->dump_step == 0) throw PisError(PIS_ERR_INVALID_ARGUMENT, "dump_step must be positive");
+            dumper.reset(new DumpTraj(*dump));
+            dumper->write_step(atoms, 0);
+        }
         std::fill(atoms.forces.begin(), atoms.forces.end(), 0.0);
         const double first = mgr.compute_potential(atoms);
-        if (out) std::fprintf(out, "0 %.17g\n", first);
-        std::vector<pis_thermo> th;
-        mgr.run_nve(atoms, dt, steps, &th);
-        if (out)
-            for (uint64_t i = 0; i < steps; ++i)
-                std::fprintf(out, "%llu %.3f %.3f %.3f %.3f %.3f\n", (unsigned long long)(i + 1), th[i].pe,
-                             th[i].ke, th[i].etot, th[i].temperature, th[i].pressure);
-        return th;
+        if (out) std::fprintf(out, "0 %s\n", rust_display(first).c_str());
+        if (ensemble == Ensemble::NVT) mgr.nvt_init(atoms, chain);
+        if (ensemble == Ensemble::NPT) mgr.npt_init(atoms, chain, barostat);
+        const uint64_t cadence = dump ? dump->dump_step : (steps ? steps : 1);
+        std::vector<pis_thermo> all, th;
+        uint64_t done = 0;
+        while (done < steps) {
+            const uint64_t k = std::min<uint64_t>(cadence - done % cadence, steps - done);
+            if (ensemble == Ensemble::NVT)
+                mgr.run_nvt(atoms, dt, k, done, steps, &th);
+            else if (ensemble == Ensemble::NPT)
+                mgr.run_npt(atoms, dt, k, done, steps, &th);
+            else
+                mgr.run_nve(atoms, dt, k, &th);
+            if (out)
+                for (uint64_t j = 0; j < k; ++j)
+                    std::fprintf(out, "%llu %.3f %.3f %.3f %.3f %.3f\n", (unsigned long long)(done + j + 1),
+                                 th[j].pe, th[j].ke, th[j].etot, th[j].temperature, th[j].pressure);
+            all.insert(all.end(), th.begin(), th.end());
+            done += k;
+            if (dumper && done % cadence == 0) dumper->write_step(atoms, done);
+        }
+        return all;
     }
 };
 
