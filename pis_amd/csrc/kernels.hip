@@ -988,7 +988,10 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
                     w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
                     w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
                     w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
-                    w3 = __builtin_amdgcn_alignbit((base + b) << 3, w3, 16);
+                    // entry: LDS byte offset of the slot's doubles (a multiple of 8), with the
+                    // partner's type index in the three low bits (mixtures)
+                    const uint32_t sl = base + b;
+                    w3 = __builtin_amdgcn_alignbit((sl << 3) | (MULTI ? (uint32_t)Pt[sl] : 0u), w3, 16);
                     ++cnt;
                     if ((cnt & 7) == 0 && (cnt >> 3) <= maxg)
                         out[(size_t)((cnt >> 3) - 1) * npad] = u32x4{w0, w1, w2, w3};
@@ -1166,9 +1169,10 @@ __device__ __forceinline__ void atom_forces(const SysDev& s, const double* Px, c
             for (int e = 0; e < B; ++e) {
                 const int k = h + e;
                 off[e] = (wv[k >> 1] >> ((k & 1) * 16)) & 0xffffu;
-                X[e] = *reinterpret_cast<const double*>(bx + off[e]);
-                Y[e] = *reinterpret_cast<const double*>(by + off[e]);
-                Z[e] = *reinterpret_cast<const double*>(bz + off[e]);
+                const uint32_t ad = MULTI ? off[e] & ~7u : off[e];  // low bits: partner type
+                X[e] = *reinterpret_cast<const double*>(bx + ad);
+                Y[e] = *reinterpret_cast<const double*>(by + ad);
+                Z[e] = *reinterpret_cast<const double*>(bz + ad);
             }
             double R2[B];
             bool IN[B];
@@ -1187,7 +1191,7 @@ __device__ __forceinline__ void atom_forces(const SysDev& s, const double* Px, c
                 Y[e] = dy;
                 Z[e] = dz;
                 R2[e] = fma(dx, dx, fma(dy, dy, dz * dz));
-                const PairParam& p = MULTI ? ppl[ti * s.ntypes + Pt[off[e] >> 3]] : p0;
+                const PairParam& p = MULTI ? ppl[ti * s.ntypes + (off[e] & 7u)] : p0;
                 const bool valid = h + e < rem;
                 if (MIXED) {
                     IN[e] = valid && R2[e] <= p.r2_force;
@@ -1200,7 +1204,7 @@ __device__ __forceinline__ void atom_forces(const SysDev& s, const double* Px, c
                 // the reference's `norm() > rcut` settled exactly inside the rounding band
 #pragma unroll
                 for (int e = 0; e < B; ++e) {
-                    const PairParam& p = MULTI ? ppl[ti * s.ntypes + Pt[off[e] >> 3]] : p0;
+                    const PairParam& p = MULTI ? ppl[ti * s.ntypes + (off[e] & 7u)] : p0;
                     if (h + e < rem && !IN[e] && R2[e] <= p.f_hi)
                         IN[e] = exact_within<TRI>(*ex.s, *ex.st,
                                              slot_atom(ex.pref, ex.ecell, ex.ne, (int)(off[e] >> 3)), ex.xi,
@@ -1209,7 +1213,7 @@ __device__ __forceinline__ void atom_forces(const SysDev& s, const double* Px, c
             }
 #pragma unroll
             for (int e = 0; e < B; ++e) {
-                const PairParam& p = MULTI ? ppl[ti * s.ntypes + Pt[off[e] >> 3]] : p0;
+                const PairParam& p = MULTI ? ppl[ti * s.ntypes + (off[e] & 7u)] : p0;
                 lj_accum<MIXED>(p, X[e], Y[e], Z[e], R2[e], IN[e], acc);
             }
         }
