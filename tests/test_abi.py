@@ -41,8 +41,8 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version_and_status_strings():
     lib = pis_amd.lib()
     assert lib.pis_abi_version() == 1
-    for code in range(12):
-        assert lib.pis_status_string(code)
+    for code in range(14):
+        assert lib.pis_status_string(code) != b"unknown status"
 
 
 def plan_status(spec, **kw):
@@ -66,15 +66,17 @@ def test_plan_error_mapping_mirrors_pis_error():
     assert s == 5
 
 
-def test_general_boxes_plan_with_raw_staging():
-    # a tilted box (xy) is planned with raw positions and a per-pair image; cells follow
-    # the perpendicular widths, so a tilt that thins a side below 3 cells stages the box
+def test_general_boxes_plan_tile_relative_staging():
+    # a tilted box (xy) stages tile-relative positions (its tiles leave two cells between
+    # a tile and its periodic image); cells follow the perpendicular widths, so a tilt that
+    # thins a side below 3 cells stages the whole box with a per-pair image
     big = W.lj_liquid(8, 0.8442, 0.72, seed=1)
     h = list(big.box.h)
     h[3] = 0.1 * h[0]
     big.box = pis_amd.SimulationBox(h)
     st, info = plan_status(big)
-    assert st == 0 and info.mode == 1
+    assert st == 0 and info.mode == 0
+    assert all(info.ncell[k] - info.tile[k] >= 2 for k in range(3))
     L = big.box.h[0]
     assert list(info.ncell)[:2] == [int(L / (2.5 + 0.3)), int(L / (2.5 + 0.3))]
     small = W.lj_liquid(4, 0.8442, 0.72, seed=1)
