@@ -1418,12 +1418,23 @@ __device__ __forceinline__ void sum_partials(const SysDev& s, double* v, double*
 {
     v[0] = v[1] = v[2] = v[3] = 0.0;
     const __attribute__((address_space(1))) double* P = gbl(static_cast<const double*>(s.partial));
-    for (int t = threadIdx.x; t < s.ntile; t += blockDim.x) {
-        v[0] += P[4 * t + 0];
-        v[1] += P[4 * t + 1];
-        v[2] += P[4 * t + 2];
-        v[3] += P[4 * t + 3];
+    // four tiles' partials in flight per thread (the loop is latency-bound otherwise)
+    const int bd = blockDim.x;
+    int t = threadIdx.x;
+    for (; t + 3 * bd < s.ntile; t += 4 * bd) {
+        double q[4][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) q[u][k] = P[4 * (t + u * bd) + k];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[k] += q[u][k];
     }
+    for (; t < s.ntile; t += bd)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] += P[4 * t + k];
     block_sum4(v, rtmp);
 }
 
