@@ -88,7 +88,10 @@ char* put_u64(char* p, uint64_t v)
 
 constexpr size_t kLineMax = 24 + 1 + 12 + 3 * (1 + 330) + 1;
 
-// "id type x y z\n" for atoms [a, b) into out (grown as needed).
+// "id type x y z\n" for atoms [a, b) into out (grown as needed); false if memory ran out
+// (a worker thread must not let an exception escape).
+bool format_atoms_into(const double* pos, const uint32_t* type_ids, int64_t a, int64_t b, std::string& out);
+
 void format_atoms(const double* pos, const uint32_t* type_ids, int64_t a, int64_t b, std::string& out)
 {
     out.clear();
@@ -109,6 +112,17 @@ void format_atoms(const double* pos, const uint32_t* type_ids, int64_t a, int64_
         used += (size_t)(q - p);
     }
     out.resize(used);
+}
+
+bool format_atoms_into(const double* pos, const uint32_t* type_ids, int64_t a, int64_t b, std::string& out)
+{
+    try {
+        format_atoms(pos, type_ids, a, b, out);
+        return true;
+    } catch (...) {
+        out.clear();
+        return false;
+    }
 }
 
 bool write_all(int fd, const char* p, size_t len)
@@ -165,16 +179,22 @@ extern "C" pis_status pis_dump_write_block(int fd, uint64_t step, uint64_t n_ato
     const int64_t per = std::max<int64_t>(4096, (n + nt - 1) / std::max(1, nt));
     const int chunks = (int)std::max<int64_t>(1, (n + per - 1) / per);
     std::vector<std::string> parts((size_t)chunks);
+    std::vector<char> ok((size_t)chunks, 0);
+    std::vector<std::thread> pool;
+    bool started = true;
     try {
-        std::vector<std::thread> pool;
         pool.reserve((size_t)chunks - 1);
         for (int c = 1; c < chunks; ++c)
-            pool.emplace_back(format_atoms, pos, type_ids, c * per, std::min(n, (c + 1) * per), std::ref(parts[c]));
-        format_atoms(pos, type_ids, 0, std::min(n, per), parts[0]);
-        for (auto& t : pool) t.join();
-    } catch (const std::exception& ex) {  // thread start or buffer growth failed
-        return pis::fail(PIS_ERR_IO, std::string("trajectory formatting failed: ") + ex.what());
+            pool.emplace_back([&, c] {
+                ok[c] = format_atoms_into(pos, type_ids, c * per, std::min(n, (c + 1) * per), parts[c]);
+            });
+    } catch (...) {  // a thread did not start: the started ones are joined below
+        started = false;
     }
+    ok[0] = format_atoms_into(pos, type_ids, 0, std::min(n, per), parts[0]);
+    for (auto& t : pool) t.join();
+    if (!started || std::find(ok.begin(), ok.end(), 0) != ok.end())
+        return pis::fail(PIS_ERR_IO, "trajectory formatting failed (threads or memory)");
     if (!write_all(fd, head.data(), head.size()))
         return pis::fail(PIS_ERR_IO, std::string("Failed to write trajectory block: ") + std::strerror(errno));
     for (const auto& s : parts)

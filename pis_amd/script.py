@@ -516,6 +516,20 @@ class AsyncFrames:
         for b in self.buf:
             b.free()
 
+    def abort(self) -> None:
+        """After an error: no further frame is written; the writer and the transfer in
+        flight finish before the pinned buffers are freed."""
+        if self.writer is not None:
+            self.writer.join()
+            self.writer = None
+        self.pending = None
+        try:
+            self.eng.snapshot_wait()
+        except Exception:
+            pass
+        for b in self.buf:
+            b.free()
+
 
 def thermo_line(step: int, th) -> str:
     """`{} {:.3} {:.3} {:.3} {:.3} {:.3}` of step, PE, KE, H, T, P (simulation.rs:83-86)."""
@@ -533,23 +547,34 @@ def run(ctx: Context, precision: str = "fp64", skin: float = 0.3, print_fn=print
     spec = to_spec(ctx, skin)
     nd = max(1, ctx.dump_args.dump_step)
     dumper = DumpTraj(os.path.join(ctx.base_dir, ctx.dump_args.file_name)) if dump else None
-    frames = None
     out = []
-    with Engine(spec, precision) as eng:
-        eng.set_state(0, spec.positions, spec.velocities)
+    try:
+        with Engine(spec, precision) as eng:
+            eng.set_state(0, spec.positions, spec.velocities)
+            if dumper is not None:
+                dumper.write_step(spec.positions, spec.type_ids, spec.box, 0)
+            pe0 = float(eng.compute_potential()[0])
+            print_fn(f"0 {rust_float(pe0)}")
+            nh = ctx.nh_chain_args
+            if ens == "nvt":
+                eng.nvt_init(nh["start_temperature"], nh["end_temperature"], nh["tau"], 3)
+            elif ens == "npt":
+                # MTKBarostat::new_from_args (npt.rs:66-87): the chain's start temperature
+                mtk = ctx.mtk_barostat_args
+                eng.npt_init(nh["start_temperature"], nh["end_temperature"], nh["tau"], mtk["start_pressure"],
+                             mtk["tau"], 3)
+            _run_chunks(eng, ctx, ens, spec, nd, dumper, print_fn, out)
+    finally:
         if dumper is not None:
-            dumper.write_step(spec.positions, spec.type_ids, spec.box, 0)
-        pe0 = float(eng.compute_potential()[0])
-        print_fn(f"0 {rust_float(pe0)}")
-        nh = ctx.nh_chain_args
-        if ens == "nvt":
-            eng.nvt_init(nh["start_temperature"], nh["end_temperature"], nh["tau"], 3)
-        elif ens == "npt":
-            # MTKBarostat::new_from_args (npt.rs:66-87): the chain's start temperature
-            mtk = ctx.mtk_barostat_args
-            eng.npt_init(nh["start_temperature"], nh["end_temperature"], nh["tau"], mtk["start_pressure"],
-                         mtk["tau"], 3)
-        done = 0
+            dumper.close()
+    return np.concatenate(out) if out else np.zeros((0, 7))
+
+
+def _run_chunks(eng, ctx, ens, spec, nd, dumper, print_fn, out):
+    """The steps in chunks ending at every dump_step: thermo lines, trajectory frames."""
+    frames = None
+    done = 0
+    try:
         while done < ctx.steps:
             k = min(nd - done % nd, ctx.steps - done)
             if ens == "nvt":
@@ -569,11 +594,12 @@ def run(ctx: Context, precision: str = "fp64", skin: float = 0.3, print_fn=print
                 if frames is None:
                     frames = AsyncFrames(eng, dumper, spec.type_ids)
                 frames.frame(done, box)
+    except BaseException:
         if frames is not None:
-            frames.close()
-    if dumper is not None:
-        dumper.close()
-    return np.concatenate(out) if out else np.zeros((0, 7))
+            frames.abort()
+        raise
+    if frames is not None:
+        frames.close()
 
 
 __all__ = ["PisScriptError", "Context", "read_script", "read_data", "contextualize", "to_spec", "run",

@@ -108,3 +108,27 @@ def test_position_snapshots_are_taken_behind_queued_steps():
             eng.snapshot_positions(1, buf[0])  # wrong size
         for p in (*buf, other):
             p.free()
+
+
+def test_script_run_stopped_mid_run_leaves_whole_frames(tmp_path):
+    """An error inside the step loop (here the thermo printer raising at step 23) ends the
+    run: the frame whose positions are still in flight is dropped, the writer of the
+    previous one finishes, and the trajectory holds only complete blocks."""
+    inp = tmp_path / "in.pis"
+    inp.write_text(f"timestep 0.01\nread_data {os.path.join(GOLDEN, 'example.txt')}\n"
+                   "velocity all create 100.0 7\ndump d all atom 5 traj.lammpstrj\nrun 40\n")
+    ctx = S.contextualize(S.read_script(str(inp), base_dir=str(tmp_path)))
+    lines = []
+
+    def printer(line):
+        lines.append(line)
+        if len(lines) == 24:  # "0 PE", then steps 1..23
+            raise RuntimeError("stop")
+
+    with pytest.raises(RuntimeError):
+        S.run(ctx, "fp64", print_fn=printer)
+    blocks = (tmp_path / "traj.lammpstrj").read_text().split("ITEM: TIMESTEP\n")[1:]
+    steps = [int(b.splitlines()[0]) for b in blocks]
+    assert steps[:4] == [0, 5, 10, 15] and len(steps) <= 5
+    for b in blocks:
+        assert len(b.splitlines()) == 1 + 2 + 1 + 3 + 1 + 500
