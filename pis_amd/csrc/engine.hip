@@ -980,10 +980,13 @@ static pis_status batch_compute(pis_ctx* c, pis_thermo* out /* [nsys] or null */
 
 // One step's launches; `first`: the position-update kernel runs (a call's first step,
 // and every NVT step, whose velocity scaling needs the chain update in between).
-static pis_status enqueue_one(pis_ctx* c, bool first, bool record, bool nvt, bool timed)
+// One step.  nvt: the Nose-Hoover chain wraps it; the chain's start-of-step update runs
+// as its own launch on a call's first step and on NPT steps (whose barostat rescaled the
+// velocities since), otherwise at the end of the previous step (k_reduce).
+static pis_status enqueue_one(pis_ctx* c, bool first, bool record, bool nvt, bool timed, bool npt = false)
 {
     const int ns = (int)c->sys.size();
-    if (nvt) launch_nhc_pre(c->stream, c->d_sys, ns);
+    if (nvt && (first || npt)) launch_nhc_pre(c->stream, c->d_sys, ns);
     if (first || nvt) launch_integrate_pos(c->stream, c->d_sys, c->d_atom_work, c->n_atom_work);
     launch_rebuild_sort(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work);
     if (timed) {
@@ -1072,7 +1075,7 @@ static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_
         for (uint64_t t = 0; t < nsteps; ++t) {
             if (t == 0) launch_npt(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work, 0, rec);
             launch_npt(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work, 1, rec);
-            PIS_TRY(enqueue_one(c, t == 0, out != nullptr, true, false));
+            PIS_TRY(enqueue_one(c, t == 0, out != nullptr, true, false, true));
             launch_npt(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work, 2, rec);
             if ((t & 63) == 0) PIS_HIP(hipGetLastError());
         }

@@ -1442,7 +1442,9 @@ __device__ void sum_tens(const SysDev& s, double* v, double* tmp);
 __device__ double sum_tens_ke(const SysDev& s, double* tmp);
 
 // Start of an NVT step (potential.rs:41-45): KE of the current velocities (the last
-// force launch's partials), chain forces and half step, and the velocity scale.
+// force launch's partials), chain forces and half step, and the velocity scale.  Launched
+// on a call's first NVT step and on every NPT step; k_reduce does it for the other NVT
+// steps at the end of the step before.
 __global__ __launch_bounds__(1024) void k_nhc_pre(const SysDev* __restrict__ sys)
 {
     __shared__ double rtmp[64];
@@ -1831,6 +1833,11 @@ __global__ __launch_bounds__(1024) void k_reduce(const SysDev* __restrict__ sys,
     if (threadIdx.x != 0) return;
     const double pe = 0.5 * v[0], ke = v[1], wabs = v[2], wpair = 0.5 * v[3];
     double ham = pe + ke;
+    // control words: all read first, then written (one round trip, not one per word)
+    __attribute__((address_space(1))) int32_t* ctl = gbl(s.ctl);
+    const int c_cur = ctl[CTL_CUR], c_rb = ctl[CTL_REBUILD], c_nrb = ctl[CTL_NREBUILD];
+    const int c_step = ctl[CTL_STEP], c_next = ctl[CTL_REBUILD_NEXT];
+    const int c_remain = ctl[CTL_REMAIN], c_left = ctl[CTL_LEFT];
     if (step && s.nhc_on) {
         // end of an NVT step (potential.rs:51-54), H with the chain's energy
         // (simulation.rs:100-105; the chain coordinates eta never move, so its potential
@@ -1841,12 +1848,15 @@ __global__ __launch_bounds__(1024) void k_reduce(const SysDev* __restrict__ sys,
         ham = ham + nhc_kinetic(c) + 0.0;
         c[NHC_T] = c[NHC_T0] + ((c[NHC_T1] - c[NHC_T0]) / c[NHC_TOTAL]) * c[NHC_STEP];
         c[NHC_STEP] = c[NHC_STEP] + 1.0;
+        if (!s.npt_on && c_remain > 1 && c_left != 1) {
+            // the next step of this call starts from these same velocities, so its chain
+            // update (potential.rs:41-45, what k_nhc_pre does) is this KE's: done here,
+            // and that step launches no k_nhc_pre
+            nhc_forces(c, ke, s.n);
+            nhc_half_step(c, s.dt);
+            c[NHC_SCALE] = exp(-0.5 * s.dt * c[NHC_XI]);
+        }
     }
-    // control words: all read first, then written (one round trip, not one per word)
-    __attribute__((address_space(1))) int32_t* ctl = gbl(s.ctl);
-    const int c_cur = ctl[CTL_CUR], c_rb = ctl[CTL_REBUILD], c_nrb = ctl[CTL_NREBUILD];
-    const int c_step = ctl[CTL_STEP], c_next = ctl[CTL_REBUILD_NEXT];
-    const int c_remain = ctl[CTL_REMAIN], c_left = ctl[CTL_LEFT];
     if (record) {
         if (c_step < s.thermo_cap) {
             __attribute__((address_space(1))) double* T = gbl(reinterpret_cast<double*>(s.thermo + c_step));
