@@ -362,14 +362,14 @@ __device__ int stage_tile(const SysDev& s, const State& st, const TileGeom& g, d
             Px[sl] = (T)x;
             Py[sl] = (T)y;
             Pz[sl] = (T)z;
-            Pt[sl] = gbl(st.type)[a];
+            if (Pt) Pt[sl] = gbl(st.type)[a];  // the force kernel reads types from the list
         }
     }
     if (tid == 0 && WHAT != STAGE_INDEX) {
         Px[SCAP] = (T)0;
         Py[SCAP] = (T)0;
         Pz[SCAP] = (T)0;
-        Pt[SCAP] = 0;
+        if (Pt) Pt[SCAP] = 0;
     }
     __syncthreads();
     return total;
@@ -1142,7 +1142,7 @@ typedef __attribute__((address_space(1))) const u32x4 gu32x4;
 
 template <int SCAP, bool MIXED, bool MULTI, bool PAIRMI, bool TRI>
 __device__ __forceinline__ void atom_forces(const SysDev& s, const double* Px, const double* Py,
-                                            const double* Pz, const uint8_t* Pt,
+                                            const double* Pz,
                                             const PairParam* ppl, const PairParam& p0, int a, int ti,
                                             double qx, double qy, double qz, const ExactCtx& ex,
                                             Acc& acc)
@@ -1227,11 +1227,10 @@ __global__ __launch_bounds__(kTileThreads, KIND == KIND_TRI_RAW ? 1 : (MIXED ? P
                                                          int integrate)
 {
     __shared__ double Px[SCAP + 1], Py[SCAP + 1], Pz[SCAP + 1];
-    __shared__ uint8_t Pt[SCAP + 1];
     __shared__ int pref[kMaxHalo + 1];
     __shared__ int ecell[kMaxHalo];
     __shared__ int stmp[16];
-    __shared__ double rtmp[64];
+    __shared__ double rtmp[4 * (kTileThreads / 64)];
     __shared__ PairParam ppl[kMaxTypes * kMaxTypes];
 
     const WorkItem w = tiles[xcd_remap(blockIdx.x, ntw)];
@@ -1262,9 +1261,9 @@ __global__ __launch_bounds__(kTileThreads, KIND == KIND_TRI_RAW ? 1 : (MIXED ? P
     for (int q = threadIdx.x; q < nt * nt; q += blockDim.x) ppl[q] = s.pp[q];
     int total;
     if (rel)
-        total = stage_tile<SCAP, STAGE_REL>(s, st, g, cx, cy, cz, Px, Py, Pz, Pt, pref, ecell, stmp);
+        total = stage_tile<SCAP, STAGE_REL>(s, st, g, cx, cy, cz, Px, Py, Pz, (uint8_t*)nullptr, pref, ecell, stmp);
     else
-        total = stage_tile<SCAP, STAGE_RAW>(s, st, g, cx, cy, cz, Px, Py, Pz, Pt, pref, ecell, stmp);
+        total = stage_tile<SCAP, STAGE_RAW>(s, st, g, cx, cy, cz, Px, Py, Pz, (uint8_t*)nullptr, pref, ecell, stmp);
     double sum[4] = {0.0, 0.0, 0.0, 0.0};
     if (total <= SCAP) {
         const PairParam p0 = s.pp[0];  // uniform: scalar loads, no vector registers
@@ -1279,25 +1278,25 @@ __global__ __launch_bounds__(kTileThreads, KIND == KIND_TRI_RAW ? 1 : (MIXED ? P
             if (TRI) {  // general box: tile-relative staging, or raw with a per-pair image
                 if (KIND == KIND_TRI) {
                     if (nt == 1)
-                        atom_forces<SCAP, MIXED, false, false, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+                        atom_forces<SCAP, MIXED, false, false, true>(s, Px, Py, Pz, ppl, p0, a, ti, qx, qy, qz, ex, acc);
                     else
-                        atom_forces<SCAP, MIXED, true, false, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+                        atom_forces<SCAP, MIXED, true, false, true>(s, Px, Py, Pz, ppl, p0, a, ti, qx, qy, qz, ex, acc);
                 } else {
                     if (nt == 1)
-                        atom_forces<SCAP, MIXED, false, true, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+                        atom_forces<SCAP, MIXED, false, true, true>(s, Px, Py, Pz, ppl, p0, a, ti, qx, qy, qz, ex, acc);
                     else
-                        atom_forces<SCAP, MIXED, true, true, true>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+                        atom_forces<SCAP, MIXED, true, true, true>(s, Px, Py, Pz, ppl, p0, a, ti, qx, qy, qz, ex, acc);
                 }
             } else if (nt == 1) {
                 if (rel)
-                    atom_forces<SCAP, MIXED, false, false, false>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+                    atom_forces<SCAP, MIXED, false, false, false>(s, Px, Py, Pz, ppl, p0, a, ti, qx, qy, qz, ex, acc);
                 else
-                    atom_forces<SCAP, MIXED, false, true, false>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+                    atom_forces<SCAP, MIXED, false, true, false>(s, Px, Py, Pz, ppl, p0, a, ti, qx, qy, qz, ex, acc);
             } else {
                 if (rel)
-                    atom_forces<SCAP, MIXED, true, false, false>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+                    atom_forces<SCAP, MIXED, true, false, false>(s, Px, Py, Pz, ppl, p0, a, ti, qx, qy, qz, ex, acc);
                 else
-                    atom_forces<SCAP, MIXED, true, true, false>(s, Px, Py, Pz, Pt, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+                    atom_forces<SCAP, MIXED, true, true, false>(s, Px, Py, Pz, ppl, p0, a, ti, qx, qy, qz, ex, acc);
             }
             const double m = s.mass[ti];
             // positions again from memory (L2): not held in registers across the pair loop
