@@ -59,7 +59,9 @@ int scap_for(double need)
 int g_tile_override[3] = {0, 0, 0};
 
 // LDS bytes of a tile kernel instance with staging capacity `scap`.
-double tile_lds(int scap) { return (scap + 1) * 25.0 + 2300.0; }
+// LDS of the force kernel for a staging class: three FP64 coordinates per slot (SCAP + 1
+// slots) plus the halo tables, the pair table and the block-sum scratch (as compiled).
+double tile_lds(int scap) { return (scap + 1) * 24.0 + 4200.0; }
 
 // Tile choice: maximise lane use of the 256-thread workgroup times an occupancy factor
 // (workgroups per CU the LDS allows, capped at the 3 the register file allows), with a
