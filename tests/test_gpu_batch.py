@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
+import pis_amd
 from pis_amd import Engine, workloads as W
 from tests.conftest import force_err
 
@@ -134,3 +135,14 @@ def test_nvt_batch_equals_alone_and_graph_equals_direct():
         assert np.array_equal(th_a[:, 0, :], th_b[:, k, :])
         for a, b, c in zip(st_a[0], st_b[k], st_g[k]):
             assert np.array_equal(a, b) and np.array_equal(b, c)
+
+
+def test_tile_autotuner_times_candidates_and_restores_the_model_pick():
+    from pis_amd import tune
+
+    spec = W.lj_liquid(12, 0.8442, 0.72, seed=31)
+    model = pis_amd.plan(spec, "mixed")["tile"]
+    best, results = tune.autotune_tiles(spec, "mixed", steps=10, limit=3)
+    assert best in [r[0] for r in results] and len(results) >= 2
+    assert all(r[2] > 0 for r in results)
+    assert pis_amd.plan(spec, "mixed")["tile"] == model
