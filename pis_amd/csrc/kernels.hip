@@ -309,10 +309,14 @@ __device__ __forceinline__ TileGeom tile_geom(const SysDev& s, int t)
 __device__ __forceinline__ int wrap_cell(int c, int n) { return c < 0 ? c + n : (c >= n ? c - n : c); }
 
 enum { STAGE_RAW = 0, STAGE_REL = 1, STAGE_INDEX = 2 };
+constexpr double kPadFar = 1e30;  // squared: 1e60, finite in FP64, +inf in FP32
 
 // Stage the atoms of a tile's halo box (or of the whole system) into LDS, cell by cell
-// in halo order.  pref[e] is the first slot of halo cell e.  Slot SCAP holds finite
-// padding (list entries past an atom's count point there and are masked by count).
+// in halo order.  pref[e] is the first slot of halo cell e.  Slot SCAP is the padding
+// that list entries past an atom's count point to: with tile-relative positions it lies
+// kPadFar away, beyond any cutoff, so those entries fail the cutoff test by themselves;
+// with raw positions (a per-pair minimum image would fold it back) it sits at the origin
+// and the force loop masks those entries by count.
 // Returns the number of staged atoms (uniform over the block); > SCAP means overflow
 // and nothing was staged.
 template <int SCAP, int WHAT, typename T>
@@ -366,9 +370,10 @@ __device__ int stage_tile(const SysDev& s, const State& st, const TileGeom& g, d
         }
     }
     if (tid == 0 && WHAT != STAGE_INDEX) {
-        Px[SCAP] = (T)0;
-        Py[SCAP] = (T)0;
-        Pz[SCAP] = (T)0;
+        const T pad = WHAT == STAGE_REL ? (T)kPadFar : (T)0;
+        Px[SCAP] = pad;
+        Py[SCAP] = pad;
+        Pz[SCAP] = pad;
         if (Pt) Pt[SCAP] = 0;
     }
     __syncthreads();
@@ -1114,12 +1119,16 @@ __device__ __forceinline__ void lj_accum(const PairParam& p, double dx, double d
         acc.pe += u;
         acc.wp = fma(fp, r2, acc.wp);
     } else {
-        const float r2f = (float)r2;
+        // a pair outside the cutoff enters as r^2 = +inf: rcp gives 0, so the force factor
+        // is -0 and, with the shift masked, so is the energy (one 32-bit select each,
+        // ahead of the FP64 conversions, instead of two per 64-bit result)
+        const float r2f = in ? (float)r2 : __builtin_inff();
         const float inv = __builtin_amdgcn_rcpf(r2f);
         const float sr2 = p.sig2f * inv;
         const float sr6 = sr2 * sr2 * sr2;
-        const float fpf = in ? fmaf(p.c48f, sr6, -p.c24f) * sr6 * inv : 0.0f;
-        const float uf = in ? fmaf(fmaf(p.c4f, sr6, -p.c4f), sr6, -p.ucutf) : 0.0f;
+        const float fpf = fmaf(p.c48f, sr6, -p.c24f) * sr6 * inv;
+        const float ucf = in ? p.ucutf : 0.0f;
+        const float uf = fmaf(fmaf(p.c4f, sr6, -p.c4f), sr6, -ucf);
         const double fp = (double)fpf;
         acc.fx = fma(-fp, dx, acc.fx);
         acc.fy = fma(-fp, dy, acc.fy);
@@ -1192,7 +1201,8 @@ __device__ __forceinline__ void atom_forces(const SysDev& s, const double* Px, c
                 Z[e] = dz;
                 R2[e] = fma(dx, dx, fma(dy, dy, dz * dz));
                 const PairParam& p = MULTI ? ppl[ti * s.ntypes + (off[e] & 7u)] : p0;
-                const bool valid = h + e < rem;
+                // tile-relative staging: padding entries fail the cutoff by distance
+                const bool valid = PAIRMI ? h + e < rem : true;
                 if (MIXED) {
                     IN[e] = valid && R2[e] <= p.r2_force;
                 } else {
@@ -1205,7 +1215,7 @@ __device__ __forceinline__ void atom_forces(const SysDev& s, const double* Px, c
 #pragma unroll
                 for (int e = 0; e < B; ++e) {
                     const PairParam& p = MULTI ? ppl[ti * s.ntypes + (off[e] & 7u)] : p0;
-                    if (h + e < rem && !IN[e] && R2[e] <= p.f_hi)
+                    if ((!PAIRMI || h + e < rem) && !IN[e] && R2[e] <= p.f_hi)
                         IN[e] = exact_within<TRI>(*ex.s, *ex.st,
                                              slot_atom(ex.pref, ex.ecell, ex.ne, (int)(off[e] >> 3)), ex.xi,
                                              ex.yi, ex.zi, p.r2_force);
