@@ -93,6 +93,8 @@ def measured_traffic(natoms):
 
 
 def drift_per_atom_per_1e4(etot, n):
+    if etot.shape[0] < 2:  # a slope needs two records
+        return None
     t = np.arange(etot.shape[0], dtype=np.float64)
     slope = np.polyfit(t, etot - etot[0], 1)[0]
     return float(slope * 1e4 / n)
