@@ -490,6 +490,132 @@ double orc_compute_fulllist(orc_sys* s, const orc_list* l, int32_t check_rcut, d
     return pe / 2.0;
 }
 
+/* splitmix64's output function (Steele, Lea & Flood 2014; Vigna's constants): the entry
+ * hash of orc_list_digest. */
+uint64_t orc_mix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+/* One atom's row of build_neighbour_list (lennard_jones.rs:377-411) with rcut + rcut_add,
+ * visited in the order orc_build_list stores it (dx, dy, dz with the per-dx dedup set,
+ * then ascending j); `visit` gets every accepted j with its minimum-image vector. */
+#define ORC_ROW_BEGIN(s, c, i, rcut_add)                                                          \
+    {                                                                                             \
+        const int64_t cur_ = orc_cell_of(&(s)->box, (c)->n, &(s)->pos[3 * (i)]);                   \
+        const int64_t cx_ = cur_ % (c)->n[0], cy_ = (cur_ / (c)->n[0]) % (c)->n[1],               \
+                      cz_ = cur_ / ((c)->n[0] * (c)->n[1]);                                       \
+        int64_t seen_[27];                                                                        \
+        for (int dx_ = -1; dx_ <= 1; ++dx_) {                                                     \
+            int nseen_ = 0;                                                                       \
+            for (int dy_ = -1; dy_ <= 1; ++dy_)                                                   \
+                for (int dz_ = -1; dz_ <= 1; ++dz_) {                                             \
+                    const int64_t nb_ = cell_index(rem_euclid(cx_ + dx_, (c)->n[0]),              \
+                                                   rem_euclid(cy_ + dy_, (c)->n[1]),              \
+                                                   rem_euclid(cz_ + dz_, (c)->n[2]), (c)->n);     \
+                    int dup_ = 0;                                                                 \
+                    for (int q_ = 0; q_ < nseen_; ++q_) dup_ |= seen_[q_] == nb_;                 \
+                    if (dup_) continue;                                                           \
+                    seen_[nseen_++] = nb_;                                                        \
+                    for (int64_t b_ = (c)->start[nb_]; b_ < (c)->start[nb_ + 1]; ++b_) {          \
+                        const int64_t j = (c)->idx[b_];                                           \
+                        if (cur_ == nb_ && (i) == j) continue;                                    \
+                        double rij[3] = {(s)->pos[3 * j] - (s)->pos[3 * (i)],                     \
+                                         (s)->pos[3 * j + 1] - (s)->pos[3 * (i) + 1],             \
+                                         (s)->pos[3 * j + 2] - (s)->pos[3 * (i) + 2]};            \
+                        orc_min_image(&(s)->box, rij);                                            \
+                        double eps, sig, rc;                                                      \
+                        if (!pair_params((s), (i), j, &eps, &sig, &rc)) continue;                 \
+                        if (orc_norm(rij) > rc + (rcut_add)) continue;
+#define ORC_ROW_END \
+    }               \
+    }               \
+    }               \
+    }
+
+/* Per-atom digest of build_neighbour_list at rcut + rcut_add (the list orc_build_list
+ * returns, without storing it): count[i] = row length, hash[i] = sum mod 2^64 of
+ * orc_mix64(j) over the row -- independent of the order of the row, so it compares a
+ * list built in any order.  Rows are independent: parallel over atoms. */
+int orc_list_digest(const orc_sys* s, double rcut_add, uint32_t* count, uint64_t* hash)
+{
+    cells_t c;
+    if (build_cells(s, orc_max_rcut(&s->table) + rcut_add, &c)) return -1;
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < s->n; ++i) {
+        uint32_t k = 0;
+        uint64_t h = 0;
+        ORC_ROW_BEGIN(s, &c, i, rcut_add)
+        ++k;
+        h += orc_mix64((uint64_t)j);
+        (void)rij;
+        ORC_ROW_END
+        count[i] = k;
+        hash[i] = h;
+    }
+    free_cells(&c);
+    return 0;
+}
+
+/* LJVPBuildListManager::compute_potential (lennard_jones.rs:419-455) for the atoms in
+ * idx[0..m) only: f_out[3k..3k+2] = sum over the row of -F_ij and pe_out[k] = sum of U_ij
+ * (not halved), in the row order of orc_build_list(s, 0) -- the same bits as
+ * orc_compute_fulllist gives those atoms.  pe_out may be NULL. */
+int orc_forces_subset(const orc_sys* s, const int64_t* idx, int64_t m, double* f_out, double* pe_out)
+{
+    cells_t c;
+    if (build_cells(s, orc_max_rcut(&s->table), &c)) return -1;
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : bad)
+    for (int64_t k = 0; k < m; ++k) {
+        const int64_t i = idx[k];
+        if (i < 0 || i >= s->n) {
+            bad = 1;
+            continue;
+        }
+        double fi[3] = {0.0, 0.0, 0.0}, tpe = 0.0;
+        ORC_ROW_BEGIN(s, &c, i, 0.0)
+        double u, f[3];
+        orc_lj_pair(eps, sig, rc, s->table.shift, rij, &u, f);
+        tpe += u;
+        for (int d = 0; d < 3; ++d) fi[d] += -f[d];
+        ORC_ROW_END
+        for (int d = 0; d < 3; ++d) f_out[3 * k + d] = fi[d];
+        if (pe_out) pe_out[k] = tpe;
+    }
+    free_cells(&c);
+    return bad ? -2 : 0;
+}
+
+/* Periodic pair virial sum_{i<j} r_ij . F_ij at rcut (no reference counterpart: the
+ * reference's virial is sum_i r_i . F_i, properties.rs:49-51): per atom the full-list row
+ * of orc_build_list(s, 0) (pair function of lennard_jones.rs:32-55; F_ij . r_ij =
+ * (24 eps)(2 r12 - r6)), halved, summed over atoms in ascending order. */
+double orc_pair_virial(const orc_sys* s)
+{
+    cells_t c;
+    if (build_cells(s, orc_max_rcut(&s->table), &c)) return NAN;
+    double* part = (double*)malloc(sizeof(double) * (size_t)(s->n > 0 ? s->n : 1));
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < s->n; ++i) {
+        double w = 0.0;
+        ORC_ROW_BEGIN(s, &c, i, 0.0)
+        double u, f[3];
+        orc_lj_pair(eps, sig, rc, s->table.shift, rij, &u, f);
+        w += (f[0] * rij[0] + f[1] * rij[1]) + f[2] * rij[2];
+        ORC_ROW_END
+        part[i] = w;
+    }
+    double sum = 0.0;
+    for (int64_t i = 0; i < s->n; ++i) sum += part[i];
+    free(part);
+    free_cells(&c);
+    return 0.5 * sum;
+}
+
 /* mass_i (properties.rs:9-13). */
 static inline double mass_i(const orc_sys* s, int64_t i) { return s->mass[s->type[i] - 1]; }
 

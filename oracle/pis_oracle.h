@@ -106,6 +106,18 @@ int orc_brute_list(const orc_sys* s, double rcut_add, orc_list* out);
 double orc_compute_fulllist(orc_sys* s, const orc_list* l, int32_t check_rcut,
                             double* pe_atom);
 
+/* splitmix64's output function, the entry hash of orc_list_digest. */
+uint64_t orc_mix64(uint64_t x);
+/* Per-atom row length and order-independent row hash (sum mod 2^64 of orc_mix64(j)) of
+ * the list orc_build_list(s, rcut_add) returns, computed without storing it. */
+int orc_list_digest(const orc_sys* s, double rcut_add, uint32_t* count, uint64_t* hash);
+/* Forces (3m, AoS) and unhalved pair energies (m, may be NULL) of the atoms idx[0..m),
+ * bit-identical to what orc_compute_fulllist over orc_build_list(s, 0) gives them. */
+int orc_forces_subset(const orc_sys* s, const int64_t* idx, int64_t m, double* f_out, double* pe_out);
+
+/* Periodic pair virial sum_{i<j} r_ij . F_ij of the pairs within rcut. */
+double orc_pair_virial(const orc_sys* s);
+
 double orc_verlet_step_nve(orc_sys* s, double dt, int32_t manager);
 double orc_kinetic_energy(const orc_sys* s);
 double orc_temperature(const orc_sys* s, double ke, double kb);

@@ -102,6 +102,11 @@ def lib():
             "orc_list_free": (None, [C.POINTER(List)]),
             "orc_compute_fulllist": (C.c_double, [C.POINTER(Sys), C.POINTER(List), C.c_int32, _D]),
             "orc_verlet_step_nve": (C.c_double, [C.POINTER(Sys), C.c_double, C.c_int32]),
+            "orc_mix64": (C.c_uint64, [C.c_uint64]),
+            "orc_pair_virial": (C.c_double, [C.POINTER(Sys)]),
+            "orc_list_digest": (C.c_int, [C.POINTER(Sys), C.c_double, C.POINTER(C.c_uint32),
+                                          C.POINTER(C.c_uint64)]),
+            "orc_forces_subset": (C.c_int, [C.POINTER(Sys), C.POINTER(C.c_int64), C.c_int64, _D, _D]),
             "orc_kinetic_energy": (C.c_double, [C.POINTER(Sys)]),
             "orc_temperature": (C.c_double, [C.POINTER(Sys), C.c_double, C.c_double]),
             "orc_virial_trace": (C.c_double, [C.POINTER(Sys)]),
@@ -244,6 +249,33 @@ class OSystem:
         lib().orc_list_free(C.byref(L))
         return pe, pe_atom
 
+    def list_digest(self, rcut_add=0.0):
+        """(count, hash) per atom of the full list at rcut + rcut_add: row length and the
+        order-independent row hash sum(mix64(j)) mod 2^64 (orc_list_digest)."""
+        count = np.zeros(self.n, dtype=np.uint32)
+        h = np.zeros(self.n, dtype=np.uint64)
+        rc = lib().orc_list_digest(C.byref(self.s), rcut_add, count.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                   h.ctypes.data_as(C.POINTER(C.c_uint64)))
+        if rc != 0:
+            raise RuntimeError("oracle list digest failed")
+        return count, h
+
+    def pair_virial(self) -> float:
+        """sum_{i<j} r_ij . F_ij over the pairs within the cutoff (orc_pair_virial)."""
+        return lib().orc_pair_virial(C.byref(self.s))
+
+    def forces_subset(self, idx):
+        """(forces [m, 3], unhalved pair energies [m]) of the atoms idx, as the full-list
+        gather gives them (orc_forces_subset)."""
+        idx = np.ascontiguousarray(np.asarray(idx, dtype=np.int64))
+        f = np.zeros((idx.size, 3))
+        pe = np.zeros(idx.size)
+        rc = lib().orc_forces_subset(C.byref(self.s), idx.ctypes.data_as(C.POINTER(C.c_int64)), idx.size,
+                                     f.ctypes.data_as(_D), pe.ctypes.data_as(_D))
+        if rc != 0:
+            raise RuntimeError("oracle subset forces failed")
+        return f, pe
+
     def verlet_step_nve(self, dt, manager=MGR_OFFSET) -> float:
         return lib().orc_verlet_step_nve(C.byref(self.s), dt, manager)
 
@@ -337,6 +369,24 @@ def divide_into_cells(box: Box, rcut):
     n = np.zeros(3, dtype=np.int64)
     lib().orc_divide_into_cells(C.byref(box), rcut, n.ctypes.data_as(C.POINTER(C.c_int64)))
     return tuple(int(x) for x in n)
+
+
+def mix64(x):
+    """splitmix64's output function on a uint64 array (numpy wraps on overflow) -- the
+    same entry hash as orc_mix64, restated independently for the digest tests."""
+    x = np.asarray(x, dtype=np.uint64) + np.uint64(0x9E3779B97F4A7C15)
+    x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return x ^ (x >> np.uint64(31))
+
+
+def csr_digest(start, idx):
+    """(count, hash) per row of a CSR list: row length and sum of mix64(j) mod 2^64."""
+    start = np.asarray(start, dtype=np.int64)
+    counts = np.diff(start).astype(np.uint32)
+    h = mix64(np.asarray(idx, dtype=np.uint64))
+    cs = np.concatenate([[np.uint64(0)], np.cumsum(h, dtype=np.uint64)])
+    return counts, (cs[start[1:]] - cs[start[:-1]]).astype(np.uint64)
 
 
 def pair_set(start, idx):

@@ -594,3 +594,106 @@ def test_npt_volume_follows_the_pressure_difference(p_target, grows):
     # edge for 500 atoms in 100 steps): NPT needs a general box matrix
     h = hb[-1].reshape(3, 3).T
     assert 0.0 < np.abs(h - np.diag(np.diag(h))).max() < 5e-2 * np.abs(np.diag(h)).min()
+
+
+# ------------------------------------------------------- list digest, subset forces
+# (helpers for parity checks at sizes where whole lists or whole-system oracle runs are
+# too large: each is pinned here against the stored list / full computation it replaces)
+
+
+def test_mix64_is_splitmix64():
+    # the first two outputs of SplitMix64 seeded with 0 (state advanced by the golden gamma
+    # before mixing): published test vectors of the generator
+    assert po.lib().orc_mix64(0) == 0xE220A8397B1DCDAF
+    assert po.lib().orc_mix64(0x9E3779B97F4A7C15) == 0x6E789E6AA1B965F4
+    xs = np.array([0, 1, 2, 12345, 2 ** 40 + 3], dtype=np.uint64)
+    assert [int(v) for v in po.mix64(xs)] == [po.lib().orc_mix64(int(x)) for x in xs]
+
+
+@pytest.mark.parametrize("name", ["lj_6", "ka_1500", "thin_400", "example"])
+def test_list_digest_equals_the_stored_lists_digest(name):
+    spec = {
+        "lj_6": lambda: W.lj_liquid(6, 0.8442, 0.72, seed=3),
+        "ka_1500": lambda: W.ka_mixture(1500, 1.2, 1.0, seed=2),
+        # fewer than 3 cells per side: rows carry the reference's duplicate entries
+        "thin_400": lambda: W.ka_mixture(400, 1.2, 1.0, seed=4),
+        "example": lambda: _example_spec(),
+    }[name]()
+    o = po.OSystem.from_spec(spec)
+    for add in (0.0, spec.skin):
+        count, h = o.list_digest(add)
+        st, ix = o.list(add)
+        c2, h2 = po.csr_digest(st, ix)
+        assert np.array_equal(count, c2)
+        assert np.array_equal(h, h2)
+    if name == "thin_400":
+        assert po.divide_into_cells(o.s.box, max(p[4] for p in spec.pairs) + spec.skin)[0] < 3
+
+
+def test_list_digest_detects_a_changed_row():
+    spec = W.lj_liquid(6, 0.8442, 0.72, seed=3)
+    o = po.OSystem.from_spec(spec)
+    st, ix = o.list(spec.skin)
+    c0, h0 = po.csr_digest(st, ix)
+    ix2 = ix.copy()
+    ix2[st[5]] = (ix2[st[5]] + 1) % spec.n  # one partner swapped for another atom
+    c1, h1 = po.csr_digest(st, ix2)
+    assert np.array_equal(c0, c1)
+    assert h1[5] != h0[5] and np.array_equal(np.delete(h1, 5), np.delete(h0, 5))
+
+
+def _example_spec():
+    # the reference's example/example.txt: 3 cells per side, the smallest grid without the
+    # stencil's duplicates
+    return W.read_lammps_data(os.path.join(GOLDEN, "example.txt"))
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_forces_subset_equals_the_full_list_gather(seed):
+    spec = W.lj_liquid(6, 0.8442, 0.72, seed=seed)
+    o = po.OSystem.from_spec(spec)
+    pe_full, pe_atom = o.compute_fulllist(0.0, False)
+    f_full = o.force.copy()
+    idx = np.random.default_rng(seed).choice(spec.n, 97, replace=False)
+    f, pe = o.forces_subset(idx)
+    assert np.array_equal(f, f_full[idx])  # same rows, same order of summation
+    assert np.array_equal(pe, pe_atom[idx])
+    # and the shipped half-list manager agrees to rounding
+    o.compute(po.MGR_OFFSET)
+    scale = np.sqrt((o.force ** 2).sum(1).mean())
+    assert np.abs(f - o.force[idx]).max() <= 1e-12 * scale
+
+
+def test_forces_subset_on_a_mixture_matches_brute_force():
+    spec = W.ka_mixture(1500, 1.2, 1.0, seed=6)
+    o = po.OSystem.from_spec(spec)
+    o.compute(po.MGR_BRUTE)
+    idx = np.arange(0, spec.n, 7)
+    f, _ = o.forces_subset(idx)
+    scale = np.sqrt((o.force ** 2).sum(1).mean())
+    assert np.abs(f - o.force[idx]).max() <= 1e-12 * scale
+
+
+def test_pair_virial_against_numpy_brute_force():
+    spec = W.ka_mixture(1500, 1.2, 1.0, seed=6)
+    o = po.OSystem.from_spec(spec)
+    L = np.asarray(spec.box.lengths)
+    pos = spec.positions
+    types = np.asarray(spec.type_ids)
+    eps = {(1, 1): 1.0, (1, 2): 1.5, (2, 2): 0.5}
+    sig = {(1, 1): 1.0, (1, 2): 0.8, (2, 2): 0.88}
+    tab = {(min(a, b), max(a, b)): (e, s, rc) for (a, b, e, s, rc) in spec.pairs}
+    w = 0.0
+    for i in range(spec.n - 1):
+        d = pos[i + 1:] - pos[i]
+        d -= L * np.round(d / L)
+        r2 = (d ** 2).sum(1)
+        for j_off in range(d.shape[0]):
+            j = i + 1 + j_off
+            key = (min(types[i], types[j]), max(types[i], types[j]))
+            e, s, rc = tab[key]
+            if r2[j_off] <= rc * rc:
+                sr6 = (s * s / r2[j_off]) ** 3
+                w += 24 * e * (2 * sr6 * sr6 - sr6)  # r . F for the LJ pair
+    assert eps and sig  # the KA table above documents the mixture
+    assert o.pair_virial() == pytest.approx(w, rel=1e-11)
