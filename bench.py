@@ -65,12 +65,6 @@ def force_kernel_bytes_per_atom(nbar):
     return 2.0 * nbar + 2 + 24 + 1 + 48 + 24 + 72
 
 
-def step_bytes_per_atom(nbar):
-    """Whole steady-state step: the force launch, which also advances the positions (the
-    position-update kernel runs only on a call's first step)."""
-    return force_kernel_bytes_per_atom(nbar)
-
-
 def measured_traffic(natoms):
     """HBM bytes per force launch from the committed rocprofv3 counter summary
     (FETCH_SIZE / WRITE_SIZE scaled by a streaming copy of known size), rescaled to the
@@ -92,28 +86,52 @@ def measured_traffic(natoms):
     return b * natoms / kv["atoms"]
 
 
-def drift_per_atom_per_1e4(etot, n):
-    if etot.shape[0] < 2:  # a slope needs two records
-        return None
+def b_alg(nbar):
+    """Algorithmic HBM bytes per atom-step of the step (the survey's byte model):
+    4 n (int32 neighbour indices of the full list) + 24 position read + 24 force write +
+    120 for the integrator (read x, v, F; write x, v) + 24 for the skin-check reference
+    position; neighbour-position gathers are served by L2/MALL and count 0."""
+    return 4.0 * nbar + 192.0
+
+
+def drift_stats(etot, n):
+    """Energy drift of an NVE record per atom: least-squares slope of H scaled to 10^4
+    steps, H_end - H_start, and the largest |H - H_start|."""
     t = np.arange(etot.shape[0], dtype=np.float64)
-    slope = np.polyfit(t, etot - etot[0], 1)[0]
-    return float(slope * 1e4 / n)
+    slope = float(np.polyfit(t, etot - etot[0], 1)[0])
+    return {
+        "fit_per_1e4_steps": abs(slope) * 1e4 / n,
+        "end_minus_start": float(abs(etot[-1] - etot[0]) / n),
+        "max_abs_dev": float(np.abs(etot - etot[0]).max() / n),
+        "steps": int(etot.shape[0]),
+        "unit": "epsilon/atom",
+    }
 
 
-def run_gpu(spec, precision, steps, warmup):
+def run_gpu(spec, precision, steps, warmup, graph, timing_steps=50):
+    """The headline measurement: W warm-up steps with the timed call's own settings
+    (thermo recorded every step, graph mode as timed), so the thermo ring and the captured
+    step graphs exist before t0; then exactly K steps timed with kernel timing off.  The
+    force kernel's launch time comes from a separate pass with HIP events around each force
+    launch on the context's stream."""
     from pis_amd import Engine
 
     eng = Engine(spec, precision)
     eng.load()
     eng.compute_potential()
     _, _, f0 = eng.get_state(0)
+    eng.set_graph_mode(graph)
     if warmup:
-        eng.run_nve(spec.dt, warmup, record=False)
-    eng.set_kernel_timing(True)
+        eng.run_nve(spec.dt, warmup, record=True)
+    r0 = eng.stats()["rebuilds"]
     t0 = time.perf_counter()
     th = eng.run_nve(spec.dt, steps, record=True)[:, 0, :]
     t1 = time.perf_counter()
+    rebuilds = eng.stats()["rebuilds"] - r0
+    eng.set_kernel_timing(True)
+    eng.run_nve(spec.dt, timing_steps, record=True)
     st = eng.stats()
+    eng.set_kernel_timing(False)
     eng.close()
     sec = t1 - t0
     return {
@@ -121,11 +139,72 @@ def run_gpu(spec, precision, steps, warmup):
         "rate": spec.n * steps / sec,
         "ms_per_step": 1e3 * sec / steps,
         "force_ms": st["force_kernel_ms"] / max(st["timed_force_launches"], 1),
-        "rebuilds": st["rebuilds"],
+        "rebuilds": rebuilds,
         "max_neighbors": st["max_neighbors"],
         "thermo": th,
         "f0": f0,
     }
+
+
+def run_drift(spec, precision, warm, steps):
+    """Energy drift over a fixed NVE span, independent of --steps: `warm` steps of
+    equilibration, then `steps` recorded steps."""
+    from pis_amd import Engine
+
+    with Engine(spec, precision) as eng:
+        eng.load()
+        eng.compute_potential()
+        eng.set_graph_mode(True)
+        eng.run_nve(spec.dt, warm, record=False)
+        h = eng.run_nve(spec.dt, steps, record=True)[:, 0, 2]
+    out = drift_stats(h, spec.n)
+    out["warmup"] = warm
+    return out
+
+
+def run_c3(spec, steps, warm):
+    """C3 (4,000,000 atoms, rc 3.0): mixed-precision throughput over `steps` steps after
+    `warm`, its energy drift over the timed span, and the FP64 path's drift over the same
+    span; the step-0 forces of both precisions are returned for the oracle check."""
+    from pis_amd import Engine
+
+    out = {}
+    forces = {}
+    for prec in ("mixed", "fp64"):
+        with Engine(spec, prec) as eng:
+            eng.load()
+            eng.compute_potential()
+            _, _, forces[prec] = eng.get_state(0)
+            eng.set_graph_mode(True)
+            eng.run_nve(spec.dt, warm, record=True)
+            r0 = eng.stats()["rebuilds"]
+            t0 = time.perf_counter()
+            h = eng.run_nve(spec.dt, steps, record=True)[:, 0, 2]
+            sec = time.perf_counter() - t0
+            rb = eng.stats()["rebuilds"] - r0
+        out[prec] = {"value": round(spec.n * steps / sec, 1), "unit": "atom-steps/s",
+                     "ms_per_step": round(1e3 * sec / steps, 5), "rebuilds": rb,
+                     "drift": drift_stats(h, spec.n)}
+    return out, forces
+
+
+def run_c1(steps):
+    """C1: the argon FCC crystal (4000 atoms, 100 K, rc 8.5125 A, dt 1 fs) for 1000 FP64
+    steps on the GPU (the CPU leg times the oracle on the same run)."""
+    from pis_amd import Engine, workloads as W
+
+    spec = W.argon_fcc_c1()
+    with Engine(spec, "fp64") as eng:
+        eng.load()
+        eng.compute_potential()
+        eng.set_graph_mode(True)
+        eng.run_nve(spec.dt, 20, record=True)
+        t0 = time.perf_counter()
+        th = eng.run_nve(spec.dt, steps, record=True)[:, 0, :]
+        sec = time.perf_counter() - t0
+    return spec, {"value": round(spec.n * steps / sec, 1), "unit": "atom-steps/s", "steps": steps,
+                  "ms_per_step": round(1e3 * sec / steps, 5), "precision": "fp64",
+                  "drift": drift_stats(th[:, 2], spec.n)}
 
 
 def run_per_call(spec, steps):
@@ -357,6 +436,33 @@ def cpu_model():
     return platform.processor() or platform.machine()
 
 
+def cpu_c1(spec, steps):
+    """The oracle's LJVOffsetManager + verlet_step_nve (1 core) on C1's full run."""
+    from oracle import pyoracle as po
+
+    o = po.OSystem.from_spec(spec)
+    t0 = time.perf_counter()
+    o.run_nve(spec.dt, steps, po.MGR_OFFSET, kb=spec.kb)
+    return spec.n * steps / (time.perf_counter() - t0)
+
+
+def c3_force_errors(spec, forces, sample, seed=2029):
+    """Force errors of both precisions at C3 against the oracle on `sample` seeded atoms
+    (the oracle's full-list gather for those atoms alone)."""
+    from oracle import pyoracle as po
+
+    idx = np.sort(np.random.default_rng(seed).choice(spec.n, sample, replace=False))
+    o = po.OSystem.from_spec(spec)
+    f_ref, _ = o.forces_subset(idx)
+    rms = math.sqrt(float((f_ref ** 2).sum(axis=1).mean()))
+    fm, fd = forces["mixed"][idx], forces["fp64"][idx]
+    return {
+        "sample_atoms": sample,
+        "force_rms_err_mixed": math.sqrt(float(((fm - f_ref) ** 2).sum(axis=1).mean())) / rms,
+        "force_max_err_fp64": float(np.abs(fd - f_ref).max() / rms),
+    }
+
+
 def cpu_ideal(spec, steps, threads):
     from oracle import pyoracle as po
 
@@ -372,9 +478,14 @@ def main():
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--config", default="c2")
+    ap.add_argument("--no-graph", action="store_true", help="direct launches instead of step-graph replay")
     ap.add_argument("--cpu-steps", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle force check")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle force checks")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--drift-steps", type=int, default=10000,
+                    help="recorded NVE steps of the drift legs, after 300 of equilibration (0: skip)")
+    ap.add_argument("--c3-steps", type=int, default=2000, help="steps of the 4M-atom C3 leg (0: skip)")
+    ap.add_argument("--c1-steps", type=int, default=1000, help="steps of the C1 argon leg (0: skip)")
     ap.add_argument("--per-call-steps", type=int, default=20,
                     help="steps of the PCIe-inclusive per-call path (0: skip)")
     ap.add_argument("--nvt-steps", type=int, default=2000, help="steps of the NVT leg (0: skip)")
@@ -395,20 +506,44 @@ def main():
               file=sys.stderr)
         return 2
 
+    graph = not args.no_graph
     spec, desc = workload(args.config)
     nbar = list_mean(spec)
     log(f"{args.config}: N={spec.n}, mean list length {nbar:.1f}")
 
-    log(f"mixed precision: warmup {args.warmup}, timed {args.steps} steps")
-    mixed = run_gpu(spec, "mixed", args.steps, args.warmup)
+    log(f"mixed precision: warmup {args.warmup}, timed {args.steps} steps, graph replay {graph}")
+    mixed = run_gpu(spec, "mixed", args.steps, args.warmup, graph)
     log(f"mixed: {mixed['rate']:.4e} atom-steps/s, {mixed['ms_per_step']:.4f} ms/step, "
         f"force kernel {mixed['force_ms']:.4f} ms, rebuilds {mixed['rebuilds']}")
     fp64 = None
     if not args.no_fp64:
         log(f"fp64: warmup {args.warmup}, timed {args.steps} steps")
-        fp64 = run_gpu(spec, "fp64", args.steps, args.warmup)
+        fp64 = run_gpu(spec, "fp64", args.steps, args.warmup, graph)
         log(f"fp64: {fp64['rate']:.4e} atom-steps/s, {fp64['ms_per_step']:.4f} ms/step, "
-            f"force kernel {fp64['force_ms']:.4f} ms")
+            f"force kernel {fp64['force_ms']:.4f} ms, rebuilds {fp64['rebuilds']}")
+
+    drift = None
+    if args.drift_steps:
+        drift = {}
+        for prec in ("mixed", "fp64"):
+            log(f"drift leg ({prec}): 300 + {args.drift_steps} NVE steps")
+            drift[prec] = run_drift(spec, prec, 300, args.drift_steps)
+            log(f"drift {prec}: fit {drift[prec]['fit_per_1e4_steps']:.3e}, end-start "
+                f"{drift[prec]['end_minus_start']:.3e} eps/atom over {args.drift_steps} steps")
+
+    c3 = c3_forces = c3_spec = None
+    if args.c3_steps and args.config == "c2":
+        c3_spec, _ = workload("c3")
+        log(f"C3 leg: N={c3_spec.n}, rc 3.0, {args.c3_steps} steps per precision")
+        c3, c3_forces = run_c3(c3_spec, args.c3_steps, 20)
+        log(f"C3 mixed: {c3['mixed']['value']:.4e} atom-steps/s ({c3['mixed']['ms_per_step']} ms/step), "
+            f"fp64 {c3['fp64']['value']:.4e}")
+
+    c1 = c1_spec = None
+    if args.c1_steps:
+        log(f"C1 leg: argon crystal, {args.c1_steps} FP64 steps on the GPU")
+        c1_spec, c1 = run_c1(args.c1_steps)
+        log(f"C1 GPU: {c1['value']:.4e} atom-steps/s, {c1['ms_per_step']} ms/step")
 
     per_call = None
     if args.per_call_steps:
@@ -454,19 +589,25 @@ def main():
         copy_gbs = 2.0 * args.copy_bytes / (ms.value * 1e-3) / 1e9
         log(f"streaming copy of {args.copy_bytes} B: {copy_gbs:.0f} GB/s (read+write)")
 
+    balg = b_alg(nbar)
     fb = force_kernel_bytes_per_atom(nbar)
-    achieved = fb * spec.n / (mixed["force_ms"] * 1e-3) / 1e9
+    achieved = balg * spec.n / (mixed["force_ms"] * 1e-3) / 1e9
+    traffic = measured_traffic(spec.n) if args.config == "c2" else None
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": (round(measured_traffic(spec.n)) if measured_traffic(spec.n) and args.config == "c2" else None),
+        "traffic": round(traffic) if traffic else None,
         "traffic_source": "profiles/k_force_lj_liquid_1M_traffic.md (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-kernel scaled)",
         "kernel": "k_force (mixed)",
-        "bytes_per_atom": round(fb, 1),
+        "bytes_per_atom": round(balg, 1),
+        "bytes_model": "B_alg = 4 n + 192 per atom-step (int32 full list, position, force, integrator, skin reference)",
         "force_kernel_ms": round(mixed["force_ms"], 5),
+        "kernel_own_bytes_per_atom": round(fb, 1),
+        "kernel_own_frac": round(fb * spec.n / (mixed["force_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "step_frac": round(balg * spec.n / (mixed["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "copy_kernel_GBs": round(copy_gbs, 1) if copy_gbs else None,
     }
 
@@ -489,6 +630,7 @@ def main():
                        f"LJVOffsetManager + verlet_step_nve restatement ({t_steps:.1f} s), {cpu_model()}"),
         }
         log(f"CPU oracle: {cpu_rate:.4e} atom-steps/s (1 core)")
+        # the box's CPU share for one GPU is 16 threads, whatever the host has
         ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
         ncores = min(ncores or 1, 16)
         ideal = cpu_ideal(spec, args.cpu_steps, ncores)
@@ -500,7 +642,17 @@ def main():
             "sample": f"{args.cpu_steps} NVE steps incl. the initial list build",
         }
         out_extra["host_cpu"] = cpu_model()
+        out_extra["host_cpus_visible"] = os.cpu_count()
         log(f"CPU ideal parallel: {ideal:.4e} atom-steps/s on {ncores} threads")
+        if c3 is not None:
+            log("C3 force errors against the oracle on 20000 sampled atoms")
+            c3["force_errors_vs_oracle"] = c3_force_errors(c3_spec, c3_forces, 20000)
+            log(f"C3: {c3['force_errors_vs_oracle']}")
+        if c1 is not None:
+            log(f"C1 CPU oracle: {args.c1_steps} steps on 1 core")
+            c1["cpu_oracle_1core"] = round(cpu_c1(c1_spec, args.c1_steps), 1)
+            c1["gpu_over_cpu"] = round(c1["value"] / c1["cpu_oracle_1core"], 1)
+            log(f"C1 CPU: {c1['cpu_oracle_1core']:.4e} atom-steps/s")
 
     nsteps = args.steps
     res = {
@@ -521,18 +673,18 @@ def main():
             "n_atoms": spec.n,
             "mean_list_length": round(nbar, 2),
             "precision": "mixed",
+            "graph_replay": graph,
         },
         "fp64_value": round(fp64["rate"], 1) if fp64 else None,
         "fp64_ms_per_step": round(fp64["ms_per_step"], 5) if fp64 else None,
+        "fp64_force_kernel_ms": round(fp64["force_ms"], 5) if fp64 else None,
         "mixed_over_fp64": round(mixed["rate"] / fp64["rate"], 3) if fp64 else None,
-        "nve_drift_per_atom_per_1e4_steps": {
-            "mixed": drift_per_atom_per_1e4(mixed["thermo"][:, 2], spec.n),
-            "fp64": drift_per_atom_per_1e4(fp64["thermo"][:, 2], spec.n) if fp64 else None,
-        },
-        "rebuilds": {"mixed": mixed["rebuilds"], "fp64": fp64["rebuilds"] if fp64 else None},
-        "step_bytes_per_atom": round(step_bytes_per_atom(nbar), 1),
+        "nve_drift": drift,
+        "rebuilds_in_timed_steps": {"mixed": mixed["rebuilds"], "fp64": fp64["rebuilds"] if fp64 else None},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "c3_lj_liquid_4M": c3,
+        "c1_argon_crystal": c1,
         "thermostat_barostat": thermostats,
         "trajectory_output": dump,
         "batched_ensemble": ensemble,

@@ -166,6 +166,13 @@ pis_status pis_verlet_step_nve(pis_ctx* ctx, double dt, uint64_t nsteps, pis_the
 pis_status pis_get_neighbor_list(pis_ctx* ctx, uint64_t* offsets, uint32_t* idx, uint64_t cap,
                                  uint64_t* total);
 
+/* Rebuild the neighbour list at the current positions and return, per atom in original
+ * order, its length (counts[n]) and the order-independent row hash hashes[n] = sum over the
+ * row of mix64(j) mod 2^64 (j the partner's original index; mix64 is splitmix64's output
+ * function).  Compares the pair set of build_neighbour_list (lennard_jones.rs:344-416) at
+ * sizes where the full list is too large to move: no reference counterpart. */
+pis_status pis_get_list_digest(pis_ctx* ctx, uint32_t* counts, uint64_t* hashes);
+
 pis_status pis_get_stats(pis_ctx* ctx, pis_stats* out);
 /* Replay one captured hipGraph per step in pis_verlet_step_nve / pis_batch_verlet_step_nve
  * (1) instead of launching each kernel (0, the default).  Ignored while kernel timing is on. */

@@ -167,6 +167,7 @@ SIGNATURES = {
         C.c_int,
         [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint64)],
     ),
+    "pis_get_list_digest": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
     "pis_get_stats": (C.c_int, [_P, C.POINTER(Stats)]),
     "pis_set_kernel_timing": (C.c_int, [_P, C.c_int32]),
     "pis_set_graph_mode": (C.c_int, [_P, C.c_int32]),

@@ -65,12 +65,14 @@ enum {
 enum { TENS_K = 0, TENS_W = 9, TENS_KE = 18, TENS_COUNT = 20 };
 
 enum { MODE_TILE_REL = 0, MODE_RAW_MI = 1, MODE_WHOLE = 2 };
-// kinds of systems a launch covers: the build kernel comes in an orthorhombic and a
-// general-box instantiation, the force kernel also splits general boxes into tile-relative
-// staging (KIND_TRI) and raw staging with a per-pair image (KIND_TRI_RAW: whole-box systems),
-// whose registers would otherwise lower the occupancy of the common case; each takes only
-// its own systems' tiles
-enum { KIND_ORTHO = 1, KIND_TRI = 2, KIND_TRI_RAW = 4 };
+// Kinds of systems a launch covers, by box shape and staging mode: tile-relative staging of
+// an orthorhombic (KIND_ORTHO) or general box (KIND_TRI), and raw staging with a per-pair
+// image (KIND_ORTHO_RAW, KIND_TRI_RAW: boxes too small for tile-relative staging, whole-box
+// systems).  The build kernel comes in an orthorhombic and a general-box instantiation; the
+// force kernel has one per kind, so the common tile-relative kinds keep their registers and
+// LDS (FP32 staging in mixed precision) free of the raw kinds' needs.  Tiles are grouped by
+// kind, and each launch takes only its own kind's tiles.
+enum { KIND_ORTHO = 1, KIND_TRI = 2, KIND_TRI_RAW = 4, KIND_ORTHO_RAW = 8 };
 
 struct PairParam {
     double r2_force;  // largest r^2 with sqrt_rn(r^2) <= rcut (-1: pair absent)
@@ -84,7 +86,8 @@ struct PairParam {
     double c24;       // 24 eps
     double c48;       // 48 eps
     double ucut;      // energy shift U(rcut) (0 without shift)
-    float sig2f, c4f, c24f, c48f, ucutf, pad;
+    float sig2f, c4f, c24f, c48f, ucutf;
+    float r2f;        // FP32 cutoff of the mixed pair loop (rcut^2 rounded to FP32)
     // list-band edges for r^2 from the list build's FP32 staging (wider: the staged
     // coordinates carry FP32 rounding); -1 for an absent pair
     float l_lo32, l_hi32;
@@ -156,7 +159,28 @@ struct SysDev {
     double* npt;           // [NPT_COUNT] barostat state
     double* tens;          // [nblk * TENS_COUNT] per-atom-block partial sums
     int32_t nblk;          // 256-atom blocks (npad / 256, rounded up)
+    int32_t list_shift;    // list entry = (LDS slot << list_shift) | partner type: 3 when the
+                           // force kernel stages FP64 coordinates (8-byte slots), 2 for FP32
 };
+
+// Cell numbering.  Cells are numbered tile-major: tiles in x-fastest order, and inside a
+// tile its cells in x-fastest order, so the atoms of a tile are contiguous after the sort.
+// First sorted cell id of tile (tx, ty, tz): the cells of every earlier tile (full z-slabs,
+// then full y-rows of this slab, then the tiles before it in its row).
+__host__ __device__ inline int32_t tile_first_cell(const int32_t* nc, const int32_t* tile, int tx, int ty, int tz)
+{
+    const int sz = min(tile[2], nc[2] - tz * tile[2]), sy = min(tile[1], nc[1] - ty * tile[1]);
+    return nc[0] * nc[1] * (tz * tile[2]) + nc[0] * (ty * tile[1]) * sz + (tx * tile[0]) * sy * sz;
+}
+
+// Sorted (tile-major) id of natural cell (x, y, z).
+__host__ __device__ inline int32_t sorted_cell_id(const int32_t* nc, const int32_t* tile, int x, int y, int z)
+{
+    const int tx = x / tile[0], ty = y / tile[1], tz = z / tile[2];
+    const int sx = min(tile[0], nc[0] - tx * tile[0]), sy = min(tile[1], nc[1] - ty * tile[1]);
+    return tile_first_cell(nc, tile, tx, ty, tz) + ((z - tz * tile[2]) * sy + (y - ty * tile[1])) * sx +
+           (x - tx * tile[0]);
+}
 
 // Work items of batched launches.
 struct WorkItem {

@@ -42,15 +42,32 @@ struct Plan {
     bool tri = false;
     double h[9] = {0}, hinv[9] = {0};
     double qmax = 0;
+    uint32_t nbcap_req = 0;  // list capacity the caller asked for (0: from the density)
+    double halo_scale = 1.0;  // tile halos are planned for density * halo_scale atoms
+    double grid_margin = 0;   // cells are at least rcl * (1 + grid_margin) wide
     std::vector<PairParam> pp;
     std::vector<double> mass;
     std::vector<uint8_t> type0;
 };
 
 
-pis_status make_plan(const pis_system_desc* d, Plan* p);
+// need_types = false: desc->type_ids is not read (the types arrive in device memory).
+pis_status make_plan(const pis_system_desc* d, Plan* p, bool need_types = true);
 // Smallest LDS staging class holding `need` atoms (-1 if none does).
 int stage_class_for(double need);
+
+// Offsets of every array of a system's device memory (256-byte aligned) and the total.
+struct Layout {
+    size_t st[2][11];
+    size_t r, ctl, cc, cs, n2s, tc, ac, ss, sk, sx, nbr, nnb, part, pp, m, nhc, box, npt, tens;
+    size_t total;
+};
+size_t align256(size_t x);
+Layout layout_of(const Plan& p);
+// Points a system descriptor into memory at b carved as Lo, scalar fields from the plan.
+void carve_system(const Plan& p, const Layout& Lo, char* b, SysDev& d);
+// The box as the device keeps it (BOX_COUNT doubles): h, h^-1 and the volume.
+void plan_box(const Plan& p, double* bx);
 
 struct SysHost {
     Plan plan;
@@ -58,6 +75,7 @@ struct SysHost {
     size_t mem_bytes = 0;
     ThermoDev* thermo = nullptr;
     int thermo_cap = 0;
+    bool pooled = false;  // mem and thermo belong to a caller's pool: never freed here
     SysDev dev{};
 };
 
@@ -74,11 +92,14 @@ struct pis_ctx {
     size_t sys_cap = 0;  // SysDev slots allocated at d_sys
     WorkItem* d_atom_work = nullptr;
     int n_atom_work = 0;
+    size_t atom_work_cap = 0;  // WorkItem slots allocated at d_atom_work
     WorkItem* d_tile_work = nullptr;
     int n_tile_work = 0;
+    size_t tile_work_cap = 0;  // WorkItem slots allocated at d_tile_work
     // tiles grouped by LDS staging capacity: one launch per class
     struct TileClass {
         int scap;
+        int kind;  // KIND_* of every tile in the class
         int first;
         int count;
         int lane;  // 0: `stream`, k > 0: side[k-1]
@@ -134,8 +155,12 @@ pis_status enqueue_classes(pis_ctx* c, bool build, int force);
 // KIND_* bits of the context's systems (which build/force instantiations a launch needs)
 inline int plan_kind(const Plan& p)
 {
-    return !p.tri ? KIND_ORTHO : (p.mode == MODE_TILE_REL ? KIND_TRI : KIND_TRI_RAW);
+    const bool rel = p.mode == MODE_TILE_REL;
+    return !p.tri ? (rel ? KIND_ORTHO : KIND_ORTHO_RAW) : (rel ? KIND_TRI : KIND_TRI_RAW);
 }
+// List entry shift of a system (SysDev::list_shift): FP32 staging in the force kernel for
+// mixed precision with tile-relative staging, FP64 otherwise.
+inline int plan_list_shift(const Plan& p) { return p.mixed && p.mode == MODE_TILE_REL ? 2 : 3; }
 inline int ctx_kinds(const pis_ctx* c)
 {
     int k = 0;
@@ -155,6 +180,16 @@ pis_status ctx_create_empty(bool mixed, pis_ctx** out);
 pis_status ctx_add(pis_ctx* c, Plan&& plan);
 // Removes the systems at the given indices (any order); later indices shift down.
 pis_status ctx_remove(pis_ctx* c, std::vector<size_t> idx);
+// Moves the systems at the given indices out of the context without freeing them (their
+// device memory and state stay as they are), in ascending index order.
+pis_status ctx_take(pis_ctx* c, std::vector<size_t> idx, std::vector<SysHost>* out);
+// Appends systems whose device memory is set up (taken from another context, or placed
+// with alloc_system_at) as the last indices.
+pis_status ctx_put(pis_ctx* c, std::vector<SysHost>&& add);
+// Places a planned system in caller-owned device memory (system_bytes(plan, thermo_records)
+// bytes, 256-byte aligned): arrays, zeros and tables enqueued on `st`, the thermo ring
+// right after the arrays.  The system is marked pooled.
+pis_status alloc_system_at(SysHost& h, void* mem, int thermo_records, hipStream_t st);
 // Device bytes a system with this plan occupies (its allocation plus `thermo_records`).
 size_t system_bytes(const Plan& p, int thermo_records);
 // Enqueues `nsteps` steps (record: thermo per step; nvt: the chain is applied) of every

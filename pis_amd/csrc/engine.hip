@@ -58,10 +58,12 @@ int scap_for(double need)
 // Global tile override used by the autotuner (0 = heuristic).
 int g_tile_override[3] = {0, 0, 0};
 
-// LDS bytes of a tile kernel instance with staging capacity `scap`.
-// LDS of the force kernel for a staging class: three FP64 coordinates per slot (SCAP + 1
-// slots) plus the halo tables, the pair table and the block-sum scratch (as compiled).
-double tile_lds(int scap) { return (scap + 1) * 24.0 + 4200.0; }
+// LDS of the force kernel for a staging class: three coordinates per slot (SCAP + 1 slots;
+// FP64, or FP32 in mixed precision) plus the halo tables, the pair tables and the block-sum
+// scratch (as compiled).
+double tile_lds(int scap, bool f32) { return (scap + 1) * (f32 ? 12.0 : 24.0) + 4800.0; }
+// Workgroups per CU the force kernel's registers allow (one wave per SIMD each).
+double force_wg_cap(bool f32) { return f32 ? 5.0 : 3.0; }
 
 // Tile choice: maximise lane use of the 256-thread workgroup times an occupancy factor
 // (workgroups per CU the LDS allows, capped at the 3 the register file allows), with a
@@ -73,18 +75,23 @@ void choose_tiles(Plan& p)
     double best = -1e300;
     int bt[3] = {1, 1, 1};
     const int lim[3] = {std::max(1, p.nc[0] - 2), std::max(1, p.nc[1] - 2), std::max(1, p.nc[2] - 2)};
+    // First only tiles planned into classes of at most 3072 slots: a configuration denser
+    // than the average (lattice planes on cell faces) then still has the 4096 class to grow
+    // into when its halos are measured; the largest class only if nothing else fits.
+    for (int pass = 0; pass < 2 && best == -1e300; ++pass)
     for (int tz = 1; tz <= std::min(lim[2], 8); ++tz)
         for (int ty = 1; ty <= std::min(lim[1], 8); ++ty)
             for (int tx = 1; tx <= std::min(lim[0], 8); ++tx) {
                 if ((tx + 2) * (ty + 2) * (tz + 2) > kMaxHalo) continue;
                 const double own = p.density * cellvol * tx * ty * tz;
-                const double stage = p.density * cellvol * (tx + 2) * (ty + 2) * (tz + 2);
+                const double stage = p.density * p.halo_scale * cellvol * (tx + 2) * (ty + 2) * (tz + 2);
                 const int sc = scap_for(stage * 1.1 + 48);
-                if (sc < 0) continue;
+                if (sc < 0 || (pass == 0 && sc > 3072)) continue;
                 const double passes = std::ceil((own * 1.06 + 4) / kTileThreads);
                 const double eff = own / (passes * kTileThreads);
-                const double wg = std::min(3.0, std::floor(163840.0 / tile_lds(sc)));
-                const double score = eff * (0.5 + 0.5 * wg / 3.0) - 0.004 * (stage / std::max(own, 1.0)) -
+                const double cap = force_wg_cap(p.mixed);
+                const double wg = std::min(cap, std::floor(163840.0 / tile_lds(sc, p.mixed)));
+                const double score = eff * (0.5 + 0.5 * wg / cap) - 0.004 * (stage / std::max(own, 1.0)) -
                                      0.01 * (std::max(tx, std::max(ty, tz)) - std::min(tx, std::min(ty, tz)));
                 if (score > best) {
                     best = score;
@@ -179,7 +186,10 @@ void set_bands(Plan* p)
     }
 }
 
-pis_status make_plan_impl(const pis_system_desc* d, Plan* p)
+pis_status plan_general_box(Plan* p);
+pis_status plan_grid(Plan* p, double margin);
+
+pis_status make_plan_impl(const pis_system_desc* d, Plan* p, bool need_types)
 {
     if (!d) return fail(PIS_ERR_INVALID_ARGUMENT, "null system descriptor");
     if (d->n_atoms == 0) return fail(PIS_ERR_NO_ATOMS, "No atoms defined in input file");
@@ -187,7 +197,7 @@ pis_status make_plan_impl(const pis_system_desc* d, Plan* p)
     if (d->flags != 0) return fail(PIS_ERR_INVALID_ARGUMENT, "pis_system_desc.flags must be 0");
     if (d->n_types == 0 || d->n_types > (uint32_t)kMaxTypes)
         return fail(PIS_ERR_INVALID_ATOM_TYPE, "n_types must be 1.." + std::to_string(kMaxTypes));
-    if (!d->type_ids || !d->masses) return fail(PIS_ERR_INVALID_ARGUMENT, "type_ids/masses missing");
+    if ((need_types && !d->type_ids) || !d->masses) return fail(PIS_ERR_INVALID_ARGUMENT, "type_ids/masses missing");
     if (d->n_pairs == 0 || !d->pairs)
         return fail(PIS_ERR_MISSING_POTENTIAL,
                     "Potential manager not initialized - missing pair_style or pair_coeff commands");
@@ -213,24 +223,7 @@ pis_status make_plan_impl(const pis_system_desc* d, Plan* p)
     p->kb = d->kb;
     p->mixed = d->precision == PIS_PRECISION_MIXED;
     if (p->tri) {
-        // SimulationBox::volume (simulation_box.rs:67-69): |det h|; cells follow the
-        // perpendicular widths V / |a_i x a_j| of the box
-        const double* h = d->h;
-        const double m11 = h[4] * h[8] - h[5] * h[7];
-        const double m12 = h[1] * h[8] - h[2] * h[7];
-        const double m13 = h[1] * h[5] - h[2] * h[4];
-        p->volume = std::fabs(h[0] * m11 - h[3] * m12 + h[6] * m13);
-        if (!(p->volume > 0.0)) return fail(PIS_ERR_INVALID_ARGUMENT, "box matrix is singular");
-        for (int k = 0; k < 3; ++k) {
-            const double* a = h + 3 * ((k + 1) % 3);
-            const double* b = h + 3 * ((k + 2) % 3);
-            const double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
-            p->L[k] = p->volume / std::sqrt(cx * cx + cy * cy + cz * cz);
-            p->Linv[k] = 1.0 / p->L[k];
-        }
-        p->qmax = 0.0;
-        for (int r = 0; r < 3; ++r)
-            p->qmax = std::max(p->qmax, std::fabs(h[r]) + std::fabs(h[3 + r]) + std::fabs(h[6 + r]));
+        PIS_TRY(plan_general_box(p));
     } else {
         for (int k = 0; k < 3; ++k) {
             p->L[k] = d->h[k * 3 + k];
@@ -246,8 +239,8 @@ pis_status make_plan_impl(const pis_system_desc* d, Plan* p)
         if (!(d->masses[t] > 0.0)) return fail(PIS_ERR_INVALID_ARGUMENT, "masses must be > 0");
         p->mass[t] = d->masses[t];
     }
-    p->type0.resize(p->n);
-    for (int64_t i = 0; i < p->n; ++i) {
+    p->type0.resize(need_types ? p->n : 0);
+    for (int64_t i = 0; need_types && i < p->n; ++i) {
         const uint32_t t = d->type_ids[i];
         if (t < 1 || t > (uint32_t)nt)
             return fail(PIS_ERR_INVALID_ATOM_TYPE, "Atom type " + std::to_string(t) + " out of range");
@@ -287,18 +280,51 @@ pis_status make_plan_impl(const pis_system_desc* d, Plan* p)
         q.c24f = (float)q.c24;
         q.c48f = (float)q.c48;
         q.ucutf = (float)q.ucut;
-        q.pad = 0.0f;
+        q.r2f = (float)(c.rcut * c.rcut);
         p->pp[(size_t)a * nt + b] = q;
         p->pp[(size_t)b * nt + a] = q;
         p->rcmax = std::max(p->rcmax, c.rcut);
     }
     p->rcl = p->rcmax + d->skin;
+    p->nbcap_req = d->neighbor_capacity;
+    return plan_grid(p, 1e-12);
+}
+
+// Box geometry of a general box from p->h: SimulationBox::volume (simulation_box.rs:67-69),
+// |det h|, and the perpendicular widths V / |a_i x a_j| the cells follow; qmax bounds the
+// components of a wrapped position.
+pis_status plan_general_box(Plan* p)
+{
+    const double* h = p->h;
+    const double m11 = h[4] * h[8] - h[5] * h[7];
+    const double m12 = h[1] * h[8] - h[2] * h[7];
+    const double m13 = h[1] * h[5] - h[2] * h[4];
+    p->volume = std::fabs(h[0] * m11 - h[3] * m12 + h[6] * m13);
+    if (!(p->volume > 0.0)) return fail(PIS_ERR_INVALID_ARGUMENT, "box matrix is singular");
+    for (int k = 0; k < 3; ++k) {
+        const double* a = h + 3 * ((k + 1) % 3);
+        const double* b = h + 3 * ((k + 2) % 3);
+        const double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
+        p->L[k] = p->volume / std::sqrt(cx * cx + cy * cy + cz * cz);
+        p->Linv[k] = 1.0 / p->L[k];
+    }
+    p->qmax = 0.0;
+    for (int r = 0; r < 3; ++r)
+        p->qmax = std::max(p->qmax, std::fabs(h[r]) + std::fabs(h[3 + r]) + std::fabs(h[6 + r]));
+    return PIS_OK;
+}
+
+// Cell grid, tiles, staging mode and class, rounding bands and list capacity of a plan
+// whose box geometry and pair table are set; cells are at least rcl (1 + margin) wide.
+pis_status plan_grid(Plan* p, double margin)
+{
+    p->grid_margin = margin;
     p->density = (double)p->n / p->volume;
     // Cell grid: edge strictly larger than rcut + skin (relative margin far above
     // rounding) so every pair within the list cutoff is in adjacent cells.
     bool thin = false;
     for (int k = 0; k < 3; ++k) {
-        const double nf = std::floor(p->L[k] / (p->rcl * (1.0 + 1e-12)));
+        const double nf = std::floor(p->L[k] / (p->rcl * (1.0 + margin)));
         p->nc[k] = (int)std::max(1.0, std::min(nf, 1024.0));
         p->cellw[k] = p->L[k] / p->nc[k];
         if (p->nc[k] < 3) thin = true;
@@ -331,7 +357,8 @@ pis_status make_plan_impl(const pis_system_desc* d, Plan* p)
         if (p->tri)
             for (int k = 0; k < 3; ++k)
                 if (p->nc[k] - p->tile[k] < 2) p->mode = MODE_RAW_MI;
-        const double stage = p->density * cellvol * (p->tile[0] + 2) * (p->tile[1] + 2) * (p->tile[2] + 2);
+        const double stage =
+            p->density * p->halo_scale * cellvol * (p->tile[0] + 2) * (p->tile[1] + 2) * (p->tile[2] + 2);
         p->scap = scap_for(stage * 1.1 + 48);
         if (p->scap < 0) return fail(PIS_ERR_UNSUPPORTED, "tile staging exceeds LDS capacity");
     }
@@ -339,12 +366,12 @@ pis_status make_plan_impl(const pis_system_desc* d, Plan* p)
     set_bands(p);
     p->npad = (int)((p->n + 63) / 64 * 64);
     int maxnb;
-    if (d->neighbor_capacity)
-        maxnb = (int)d->neighbor_capacity;
+    if (p->nbcap_req)
+        maxnb = (int)p->nbcap_req;
     else
         maxnb = (int)std::ceil(1.6 * p->density * vsphere) + 48;
     if (maxnb > kMaxList) {
-        if (d->neighbor_capacity)
+        if (p->nbcap_req)
             return fail(PIS_ERR_UNSUPPORTED, "neighbor_capacity above " + std::to_string(kMaxList));
         maxnb = kMaxList;
     }
@@ -353,26 +380,15 @@ pis_status make_plan_impl(const pis_system_desc* d, Plan* p)
     return PIS_OK;
 }
 
-size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
 }  // namespace
 
-pis_status pis::make_plan(const pis_system_desc* d, Plan* p) { return make_plan_impl(d, p); }
-int pis::stage_class_for(double need) { return scap_for(need); }
+size_t pis::align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // ---------------------------------------------------------------------------------
 // context
 
-namespace {
-
 // Carving of one system's allocation (offsets of every array, 256-byte aligned).
-struct Layout {
-    size_t st[2][11];
-    size_t r, ctl, cc, cs, n2s, tc, ac, ss, sk, sx, nbr, nnb, part, pp, m, nhc, box, npt, tens;
-    size_t total;
-};
-
-Layout layout_of(const Plan& p)
+Layout pis::layout_of(const Plan& p)
 {
     const size_t np = (size_t)p.npad, nc = (size_t)p.ncell;
     Layout L{};
@@ -410,53 +426,44 @@ Layout layout_of(const Plan& p)
     return L;
 }
 
-}  // namespace
+
+pis_status pis::make_plan(const pis_system_desc* d, Plan* p, bool need_types) { return make_plan_impl(d, p, need_types); }
+int pis::stage_class_for(double need) { return scap_for(need); }
 
 size_t pis::system_bytes(const Plan& p, int thermo_records)
 {
     return layout_of(p).total + align256(sizeof(ThermoDev) * (size_t)std::max(thermo_records, 0));
 }
 
-pis_status pis::alloc_system(SysHost& h)
+// Points a system descriptor into memory carved as layout_of(p) lays it out, and fills its
+// scalar fields from the plan.
+void pis::carve_system(const Plan& p, const Layout& Lo, char* b, SysDev& d)
 {
-    const Plan& p = h.plan;
     const size_t np = (size_t)p.npad;
-    const Layout Lo = layout_of(p);
-    const size_t off = Lo.total;
-    const auto& o_st = Lo.st;
-    const size_t o_r = Lo.r, o_ctl = Lo.ctl, o_cc = Lo.cc, o_cs = Lo.cs, o_n2s = Lo.n2s,
-                 o_tc = Lo.tc, o_ac = Lo.ac, o_ss = Lo.ss, o_sk = Lo.sk, o_nbr = Lo.nbr, o_nnb = Lo.nnb,
-                 o_part = Lo.part, o_pp = Lo.pp, o_m = Lo.m;
-    void* mem = nullptr;
-    PIS_HIP(hipMalloc(&mem, off));
-    h.mem = mem;
-    h.mem_bytes = off;
-    char* b = (char*)mem;
-    SysDev& d = h.dev;
     for (int s = 0; s < 2; ++s) {
         double** f[9] = {&d.st[s].x, &d.st[s].y, &d.st[s].z, &d.st[s].vx, &d.st[s].vy,
                          &d.st[s].vz, &d.st[s].fx, &d.st[s].fy, &d.st[s].fz};
-        for (int k = 0; k < 9; ++k) *f[k] = (double*)(b + o_st[s][k]);
-        d.st[s].type = (uint8_t*)(b + o_st[s][9]);
-        d.st[s].orig = (uint32_t*)(b + o_st[s][10]);
+        for (int k = 0; k < 9; ++k) *f[k] = (double*)(b + Lo.st[s][k]);
+        d.st[s].type = (uint8_t*)(b + Lo.st[s][9]);
+        d.st[s].orig = (uint32_t*)(b + Lo.st[s][10]);
     }
-    d.rx = (double*)(b + o_r);
+    d.rx = (double*)(b + Lo.r);
     d.ry = d.rx + np;
     d.rz = d.ry + np;
-    d.ctl = (int32_t*)(b + o_ctl);
-    d.cell_count = (int32_t*)(b + o_cc);
-    d.cell_start = (int32_t*)(b + o_cs);
-    d.nat2sorted = (int32_t*)(b + o_n2s);
-    d.tile_cell0 = (int32_t*)(b + o_tc);
-    d.atom_cell = (int32_t*)(b + o_ac);
-    d.scat_src = (int32_t*)(b + o_ss);
-    d.scat_key = (uint32_t*)(b + o_sk);
+    d.ctl = (int32_t*)(b + Lo.ctl);
+    d.cell_count = (int32_t*)(b + Lo.cc);
+    d.cell_start = (int32_t*)(b + Lo.cs);
+    d.nat2sorted = (int32_t*)(b + Lo.n2s);
+    d.tile_cell0 = (int32_t*)(b + Lo.tc);
+    d.atom_cell = (int32_t*)(b + Lo.ac);
+    d.scat_src = (int32_t*)(b + Lo.ss);
+    d.scat_key = (uint32_t*)(b + Lo.sk);
     d.scat_x = (double*)(b + Lo.sx);
-    d.nbr = (uint4*)(b + o_nbr);
-    d.nnb = (uint16_t*)(b + o_nnb);
-    d.partial = (double*)(b + o_part);
-    d.pp = (const PairParam*)(b + o_pp);
-    d.mass = (const double*)(b + o_m);
+    d.nbr = (uint4*)(b + Lo.nbr);
+    d.nnb = (uint16_t*)(b + Lo.nnb);
+    d.partial = (double*)(b + Lo.part);
+    d.pp = (const PairParam*)(b + Lo.pp);
+    d.mass = (const double*)(b + Lo.m);
     d.nhc = (double*)(b + Lo.nhc);
     d.nhc_on = 0;
     d.box = (double*)(b + Lo.box);
@@ -488,36 +495,82 @@ pis_status pis::alloc_system(SysHost& h)
     d.ncell = p.ncell;
     d.ntile = p.ntile;
     d.mode = p.mode;
-    // zero everything once (outside any captured sequence), then tables
-    PIS_HIP(hipMemset(mem, 0, off));
-    // natural -> sorted (tile-major) cell ids and the first sorted cell of every tile
-    std::vector<int32_t> n2s((size_t)p.ncell), tc((size_t)p.ntile + 1);
-    int32_t next = 0;
-    for (int tz = 0; tz < p.nt[2]; ++tz)
-        for (int ty = 0; ty < p.nt[1]; ++ty)
-            for (int tx = 0; tx < p.nt[0]; ++tx) {
-                const int t = (tz * p.nt[1] + ty) * p.nt[0] + tx;
-                tc[t] = next;
-                const int x0 = tx * p.tile[0], y0 = ty * p.tile[1], z0 = tz * p.tile[2];
-                const int sx = std::min(p.tile[0], p.nc[0] - x0), sy = std::min(p.tile[1], p.nc[1] - y0),
-                          sz = std::min(p.tile[2], p.nc[2] - z0);
-                for (int lz = 0; lz < sz; ++lz)
-                    for (int ly = 0; ly < sy; ++ly)
-                        for (int lx = 0; lx < sx; ++lx)
-                            n2s[((size_t)(z0 + lz) * p.nc[1] + (y0 + ly)) * p.nc[0] + (x0 + lx)] = next++;
-            }
-    tc[p.ntile] = next;
-    PIS_HIP(hipMemcpy(d.nat2sorted, n2s.data(), n2s.size() * 4, hipMemcpyHostToDevice));
-    PIS_HIP(hipMemcpy(d.tile_cell0, tc.data(), tc.size() * 4, hipMemcpyHostToDevice));
-    PIS_HIP(hipMemcpy((void*)d.pp, p.pp.data(), p.pp.size() * sizeof(PairParam), hipMemcpyHostToDevice));
-    PIS_HIP(hipMemcpy((void*)d.mass, p.mass.data(), p.mass.size() * 8, hipMemcpyHostToDevice));
-    double bx[BOX_COUNT] = {0.0};
+    d.list_shift = plan_list_shift(p);
+}
+
+// The box of a plan as the device keeps it: h, h^-1 (the caller's bits) and the volume.
+void pis::plan_box(const Plan& p, double* bx)
+{
+    for (int k = 0; k < BOX_COUNT; ++k) bx[k] = 0.0;
     for (int k = 0; k < 9; ++k) {
         bx[BOX_H + k] = p.h[k];
         bx[BOX_HINV + k] = p.hinv[k];
     }
     bx[BOX_VOL] = p.volume;
+}
+
+pis_status pis::alloc_system(SysHost& h)
+{
+    const Plan& p = h.plan;
+    const Layout Lo = layout_of(p);
+    void* mem = nullptr;
+    PIS_HIP(hipMalloc(&mem, Lo.total));
+    h.mem = mem;
+    h.mem_bytes = Lo.total;
+    SysDev& d = h.dev;
+    carve_system(p, Lo, (char*)mem, d);
+    // zero everything once (outside any captured sequence), then tables
+    PIS_HIP(hipMemset(mem, 0, Lo.total));
+    // natural -> sorted (tile-major) cell ids and the first sorted cell of every tile
+    std::vector<int32_t> n2s((size_t)p.ncell), tc((size_t)p.ntile + 1);
+    for (int z = 0; z < p.nc[2]; ++z)
+        for (int y = 0; y < p.nc[1]; ++y)
+            for (int x = 0; x < p.nc[0]; ++x)
+                n2s[((size_t)z * p.nc[1] + y) * p.nc[0] + x] = sorted_cell_id(p.nc, p.tile, x, y, z);
+    for (int t = 0; t < p.ntile; ++t)
+        tc[t] = tile_first_cell(p.nc, p.tile, t % p.nt[0], (t / p.nt[0]) % p.nt[1], t / (p.nt[0] * p.nt[1]));
+    tc[p.ntile] = p.ncell;
+    PIS_HIP(hipMemcpy(d.nat2sorted, n2s.data(), n2s.size() * 4, hipMemcpyHostToDevice));
+    PIS_HIP(hipMemcpy(d.tile_cell0, tc.data(), tc.size() * 4, hipMemcpyHostToDevice));
+    PIS_HIP(hipMemcpy((void*)d.pp, p.pp.data(), p.pp.size() * sizeof(PairParam), hipMemcpyHostToDevice));
+    PIS_HIP(hipMemcpy((void*)d.mass, p.mass.data(), p.mass.size() * 8, hipMemcpyHostToDevice));
+    double bx[BOX_COUNT];
+    plan_box(p, bx);
     PIS_HIP(hipMemcpy(d.box, bx, sizeof(bx), hipMemcpyHostToDevice));
+    return PIS_OK;
+}
+
+pis_status pis::alloc_system_at(SysHost& h, void* mem, int thermo_records, hipStream_t st)
+{
+    const Plan& p = h.plan;
+    const Layout Lo = layout_of(p);
+    h.mem = mem;
+    h.mem_bytes = system_bytes(p, thermo_records);
+    h.pooled = true;
+    SysDev& d = h.dev;
+    carve_system(p, Lo, (char*)mem, d);
+    h.thermo = (ThermoDev*)((char*)mem + Lo.total);
+    h.thermo_cap = thermo_records;
+    d.thermo = h.thermo;
+    d.thermo_cap = thermo_records;
+    PIS_HIP(hipMemsetAsync(mem, 0, Lo.total, st));
+    std::vector<int32_t> n2s((size_t)p.ncell), tc((size_t)p.ntile + 1);
+    for (int z = 0; z < p.nc[2]; ++z)
+        for (int y = 0; y < p.nc[1]; ++y)
+            for (int x = 0; x < p.nc[0]; ++x)
+                n2s[((size_t)z * p.nc[1] + y) * p.nc[0] + x] = sorted_cell_id(p.nc, p.tile, x, y, z);
+    for (int t = 0; t < p.ntile; ++t)
+        tc[t] = tile_first_cell(p.nc, p.tile, t % p.nt[0], (t / p.nt[0]) % p.nt[1], t / (p.nt[0] * p.nt[1]));
+    tc[p.ntile] = p.ncell;
+    double bx[BOX_COUNT];
+    plan_box(p, bx);
+    PIS_HIP(hipMemcpyAsync(d.nat2sorted, n2s.data(), n2s.size() * 4, hipMemcpyHostToDevice, st));
+    PIS_HIP(hipMemcpyAsync(d.tile_cell0, tc.data(), tc.size() * 4, hipMemcpyHostToDevice, st));
+    PIS_HIP(hipMemcpyAsync((void*)d.pp, p.pp.data(), p.pp.size() * sizeof(PairParam), hipMemcpyHostToDevice, st));
+    PIS_HIP(hipMemcpyAsync((void*)d.mass, p.mass.data(), p.mass.size() * 8, hipMemcpyHostToDevice, st));
+    PIS_HIP(hipMemcpyAsync(d.box, bx, sizeof(bx), hipMemcpyHostToDevice, st));
+    // the host tables go out of scope here
+    PIS_HIP(hipStreamSynchronize(st));
     return PIS_OK;
 }
 
@@ -535,6 +588,7 @@ void pis::free_ctx(pis_ctx* c)
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& s : c->sys) {
+        if (s.pooled) continue;
         if (s.mem) (void)hipFree(s.mem);
         if (s.thermo) (void)hipFree(s.thermo);
     }
@@ -563,17 +617,19 @@ void pis::free_ctx(pis_ctx* c)
 // uploads the work table.  Captured step graphs refer to the old table and are dropped.
 pis_status pis::group_tiles(pis_ctx* c)
 {
-    std::map<int, std::vector<WorkItem>> by_class;
+    // classes keyed by (staging capacity, kind): one build and one force launch each
+    std::map<std::pair<int, int>, std::vector<WorkItem>> by_class;
     c->scap = 1024;
     for (size_t k = 0; k < c->sys.size(); ++k) {
         const Plan& p = c->sys[k].plan;
         c->scap = std::max(c->scap, p.scap);
-        for (int t = 0; t < p.ntile; ++t) by_class[p.scap].push_back(WorkItem{(int32_t)k, t});
+        auto& v = by_class[{p.scap, plan_kind(p)}];
+        for (int t = 0; t < p.ntile; ++t) v.push_back(WorkItem{(int32_t)k, t});
     }
     std::vector<WorkItem> tw;
     c->classes.clear();
     for (auto& kv : by_class) {
-        c->classes.push_back({kv.first, (int)tw.size(), (int)kv.second.size(), 0});
+        c->classes.push_back({kv.first.first, kv.first.second, (int)tw.size(), (int)kv.second.size(), 0});
         tw.insert(tw.end(), kv.second.begin(), kv.second.end());
     }
     // Classes share kMaxLanes streams (the device serves a few hardware queues per
@@ -607,15 +663,21 @@ pis_status pis::group_tiles(pis_ctx* c)
         PIS_HIP(hipEventCreateWithFlags(&j, hipEventDisableTiming));
         c->join.push_back(j);
     }
-    if (c->d_tile_work) PIS_HIP(hipFree(c->d_tile_work));
-    c->d_tile_work = nullptr;
     c->n_tile_work = (int)tw.size();
-    hipError_t e = hipMalloc(&c->d_tile_work, sizeof(WorkItem) * tw.size());
-    if (e == hipSuccess)
-        e = hipMemcpy(c->d_tile_work, tw.data(), sizeof(WorkItem) * tw.size(), hipMemcpyHostToDevice);
-    if (e != hipSuccess)
-        return fail(e == hipErrorOutOfMemory ? PIS_ERR_OUT_OF_MEMORY : PIS_ERR_HIP,
-                    std::string("tile table: ") + hipGetErrorString(e));
+    if (c->tile_work_cap < tw.size()) {
+        // grow only (a free would synchronise the device): twice the need
+        if (c->d_tile_work) PIS_HIP(hipFree(c->d_tile_work));
+        c->d_tile_work = nullptr;
+        c->tile_work_cap = 0;
+        const size_t cap = 2 * tw.size();
+        PIS_HIP(hipMalloc(&c->d_tile_work, sizeof(WorkItem) * cap));
+        c->tile_work_cap = cap;
+    }
+    if (!tw.empty()) {
+        PIS_HIP(hipMemcpyAsync(c->d_tile_work, tw.data(), sizeof(WorkItem) * tw.size(), hipMemcpyHostToDevice,
+                               c->stream));
+        PIS_HIP(hipStreamSynchronize(c->stream));
+    }
     return PIS_OK;
 }
 
@@ -630,12 +692,19 @@ static pis_status relayout(pis_ctx* c)
         c->multi |= p.ntypes > 1;
         for (int64_t a = 0; a < p.n; a += kAtomThreads) aw.push_back(WorkItem{(int32_t)k, (int32_t)a});
     }
-    if (c->d_atom_work) PIS_HIP(hipFree(c->d_atom_work));
-    c->d_atom_work = nullptr;
     c->n_atom_work = (int)aw.size();
+    if (c->atom_work_cap < aw.size()) {
+        // grow only (a free would synchronise the device): twice the need
+        if (c->d_atom_work) PIS_HIP(hipFree(c->d_atom_work));
+        c->d_atom_work = nullptr;
+        c->atom_work_cap = 0;
+        PIS_HIP(hipMalloc(&c->d_atom_work, sizeof(WorkItem) * 2 * aw.size()));
+        c->atom_work_cap = 2 * aw.size();
+    }
     if (!aw.empty()) {
-        PIS_HIP(hipMalloc(&c->d_atom_work, sizeof(WorkItem) * aw.size()));
-        PIS_HIP(hipMemcpy(c->d_atom_work, aw.data(), sizeof(WorkItem) * aw.size(), hipMemcpyHostToDevice));
+        PIS_HIP(hipMemcpyAsync(c->d_atom_work, aw.data(), sizeof(WorkItem) * aw.size(), hipMemcpyHostToDevice,
+                               c->stream));
+        PIS_HIP(hipStreamSynchronize(c->stream));
     }
     if (c->sys_cap < c->sys.size()) {
         if (c->d_sys) PIS_HIP(hipFree(c->d_sys));
@@ -693,10 +762,36 @@ pis_status pis::ctx_remove(pis_ctx* c, std::vector<size_t> idx)
         const size_t k = idx[j];
         if (k >= c->sys.size()) return fail(PIS_ERR_INVALID_ARGUMENT, "system index out of range");
         SysHost& h = c->sys[k];
-        if (h.mem) PIS_HIP(hipFree(h.mem));
-        if (h.thermo) PIS_HIP(hipFree(h.thermo));
+        if (!h.pooled) {
+            if (h.mem) PIS_HIP(hipFree(h.mem));
+            if (h.thermo) PIS_HIP(hipFree(h.thermo));
+        }
         c->sys.erase(c->sys.begin() + (ptrdiff_t)k);
     }
+    return relayout(c);
+}
+
+pis_status pis::ctx_take(pis_ctx* c, std::vector<size_t> idx, std::vector<SysHost>* out)
+{
+    std::sort(idx.begin(), idx.end());
+    idx.erase(std::unique(idx.begin(), idx.end()), idx.end());
+    for (size_t k : idx)
+        if (k >= c->sys.size()) return fail(PIS_ERR_INVALID_ARGUMENT, "system index out of range");
+    PIS_HIP(hipStreamSynchronize(c->stream));
+    for (size_t k : idx) out->push_back(std::move(c->sys[k]));
+    for (size_t j = idx.size(); j-- > 0;) c->sys.erase(c->sys.begin() + (ptrdiff_t)idx[j]);
+    return relayout(c);
+}
+
+pis_status pis::ctx_put(pis_ctx* c, std::vector<SysHost>&& add)
+{
+    if (add.empty()) return PIS_OK;
+    for (auto& h : add) {
+        if (h.plan.mixed != c->mixed)
+            return fail(PIS_ERR_INVALID_ARGUMENT, "all systems of a context need the same precision");
+        c->sys.push_back(std::move(h));
+    }
+    add.clear();
     return relayout(c);
 }
 
@@ -735,11 +830,14 @@ static pis_status create_ctx(const pis_system_desc* descs, uint32_t nsys, pis_ct
 pis_status pis::ensure_xfer(pis_ctx* c, size_t bytes)
 {
     if (c->xfer_bytes >= bytes) return PIS_OK;
+    // at least twice the old size, so a stream of growing requests reallocates (and
+    // synchronises the device in hipFree) only a few times
+    const size_t grow = std::max(bytes, 2 * c->xfer_bytes);
     if (c->d_xfer) PIS_HIP(hipFree(c->d_xfer));
     c->d_xfer = nullptr;
     c->xfer_bytes = 0;
-    PIS_HIP(hipMalloc(&c->d_xfer, bytes));
-    c->xfer_bytes = bytes;
+    PIS_HIP(hipMalloc(&c->d_xfer, grow));
+    c->xfer_bytes = grow;
     return PIS_OK;
 }
 
@@ -771,9 +869,8 @@ pis_status pis::enqueue_classes(pis_ctx* c, bool build, int force)
     }
     for (const auto& q : c->classes) {
         hipStream_t st = q.lane == 0 ? c->stream : c->side[q.lane - 1];
-        if (build) launch_build_list(st, c->d_sys, c->d_tile_work + q.first, q.count, q.scap, ctx_kinds(c));
-        if (force >= 0)
-            launch_force(st, c->d_sys, c->d_tile_work + q.first, q.count, q.scap, c->mixed, force, ctx_kinds(c));
+        if (build) launch_build_list(st, c->d_sys, c->d_tile_work + q.first, q.count, q.scap, q.kind);
+        if (force >= 0) launch_force(st, c->d_sys, c->d_tile_work + q.first, q.count, q.scap, c->mixed, force, q.kind);
     }
     for (int l = 1; l < nl; ++l) {
         PIS_HIP(hipEventRecord(c->join[l - 1], c->side[l - 1]));
@@ -820,6 +917,42 @@ pis_status pis::check_flags(pis_ctx* c, int only)
     return PIS_OK;
 }
 
+// A configuration much denser than its average in places (a lattice filled only in part,
+// a cluster) can fill a tile's halo beyond the largest staging class: its tiles are planned
+// again for the measured peak, and the system moves into memory laid out for them.
+static pis_status replan_tiles_for_halo(pis_ctx* c, uint32_t k, int staged)
+{
+    SysHost& h = c->sys[k];
+    if (h.pooled)  // its memory belongs to a caller's pool, sized for the first plan
+        return fail(PIS_ERR_CAPACITY, "system " + std::to_string(k) + ": " + std::to_string(staged) +
+                                          " atoms in one tile's halo exceed the LDS staging limit");
+    SysHost nh;
+    nh.plan = h.plan;
+    Plan& p = nh.plan;
+    const double cellvol = p.cellw[0] * p.cellw[1] * p.cellw[2];
+    const double planned = p.density * p.halo_scale * cellvol * (p.tile[0] + 2) * (p.tile[1] + 2) * (p.tile[2] + 2);
+    p.halo_scale *= 1.05 * staged / std::max(planned, 1.0);
+    PIS_TRY(plan_grid(&p, p.grid_margin));
+    pis_status st = alloc_system(nh);
+    if (st != PIS_OK) {
+        if (nh.mem) (void)hipFree(nh.mem);
+        return st;
+    }
+    nh.dev.dt = h.dev.dt;
+    nh.dev.dt2 = h.dev.dt2;
+    nh.thermo = h.thermo;
+    nh.thermo_cap = h.thermo_cap;
+    nh.dev.thermo = h.thermo;
+    nh.dev.thermo_cap = h.thermo_cap;
+    // a thermostat or barostat set up before the state keeps its device state
+    PIS_HIP(hipMemcpyAsync(nh.dev.nhc, h.dev.nhc, NHC_COUNT * 8, hipMemcpyDeviceToDevice, c->stream));
+    PIS_HIP(hipMemcpyAsync(nh.dev.npt, h.dev.npt, NPT_COUNT * 8, hipMemcpyDeviceToDevice, c->stream));
+    PIS_HIP(hipStreamSynchronize(c->stream));
+    (void)hipFree(h.mem);
+    h = std::move(nh);
+    return relayout(c);
+}
+
 pis_status pis::sys_set_state(pis_ctx* c, uint32_t k, const double* pos, const double* vel,
                                 const double* force)
 {
@@ -857,9 +990,13 @@ pis_status pis::sys_set_state(pis_ctx* c, uint32_t k, const double* pos, const d
     const int staged = ctl[CTL_MAXSTAGE];
     if (p.mode != MODE_WHOLE && staged * 1.02 + 16 > p.scap) {
         const int grown = scap_for(staged * 1.1 + 48);
-        if (grown < 0)
-            return fail(PIS_ERR_CAPACITY, "system " + std::to_string(k) + ": " + std::to_string(staged) +
-                                              " atoms in one tile's halo exceed the LDS staging limit");
+        if (grown < 0) {
+            if (p.halo_scale > 1.0)  // planned for a measured peak already: give up
+                return fail(PIS_ERR_CAPACITY, "system " + std::to_string(k) + ": " + std::to_string(staged) +
+                                                  " atoms in one tile's halo exceed the LDS staging limit");
+            PIS_TRY(replan_tiles_for_halo(c, k, staged));
+            return sys_set_state(c, k, pos, vel, force);
+        }
         p.scap = grown;
         PIS_TRY(group_tiles(c));
         int32_t again[CTL_COUNT] = {0};
@@ -966,6 +1103,9 @@ static pis_status batch_compute(pis_ctx* c, pis_thermo* out /* [nsys] or null */
 {
     if (!c) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
     PIS_TRY(ensure_thermo(c, 1));
+    // a pending rebuild (the last step's skin check fired, or the staging changed) first:
+    // the list must hold every pair within the cutoff at these positions
+    PIS_TRY(run_rebuild(c));
     launch_set_ctl(c->stream, c->d_sys, (int)c->sys.size(), CTL_STEP, 0);
     PIS_TRY(launch_force_timed(c, 0));
     launch_reduce(c->stream, c->d_sys, (int)c->sys.size(), 1, 1, 0);
@@ -1038,6 +1178,62 @@ pis_status pis::enqueue_steps(pis_ctx* c, uint64_t nsteps, bool record, bool nvt
     return PIS_OK;
 }
 
+// NPT on a single-system context: the barostat would have made the box thinner than its
+// cell grid (CTL_BOX), and the system is frozen at the start of that step with its state,
+// chain and barostat as the previous step left them.  The reference bins cells afresh in
+// every force call (lennard_jones.rs:188-190, neighbour_list.rs:45-57); here the grid is
+// planned again for the current box with a 5 % margin, the system moves into memory laid
+// out for it, and the list is rebuilt.
+static pis_status replan_for_box(pis_ctx* c, uint32_t k)
+{
+    SysHost& h = c->sys[k];
+    if (h.pooled) return fail(PIS_ERR_CAPACITY, "the box deformed below the planned cell grid (rcut + skin)");
+    const size_t n = (size_t)h.plan.n;
+    double box[BOX_COUNT], npt[NPT_COUNT], chain[NHC_COUNT];
+    PIS_HIP(hipMemcpyAsync(box, h.dev.box, sizeof(box), hipMemcpyDeviceToHost, c->stream));
+    PIS_HIP(hipMemcpyAsync(npt, h.dev.npt, sizeof(npt), hipMemcpyDeviceToHost, c->stream));
+    PIS_HIP(hipMemcpyAsync(chain, h.dev.nhc, sizeof(chain), hipMemcpyDeviceToHost, c->stream));
+    PIS_HIP(hipStreamSynchronize(c->stream));
+    std::vector<double> pos(3 * n), vel(3 * n), frc(3 * n);
+    PIS_TRY(sys_get_state(c, k, pos.data(), vel.data(), frc.data()));
+    SysHost nh;
+    nh.plan = h.plan;
+    Plan& p = nh.plan;
+    for (int q = 0; q < 9; ++q) {
+        p.h[q] = box[BOX_H + q];
+        p.hinv[q] = box[BOX_HINV + q];
+    }
+    p.tri = true;
+    PIS_TRY(plan_general_box(&p));
+    double ext = 0.0;
+    for (int r = 0; r < 3; ++r) ext = std::max(ext, std::fabs(p.h[r]) + std::fabs(p.h[3 + r]) + std::fabs(p.h[6 + r]));
+    p.qmax = std::max(p.qmax, 1.25 * ext);  // the box may grow again (see npt_init)
+    PIS_TRY(plan_grid(&p, 0.05));
+    pis_status st = alloc_system(nh);
+    if (st != PIS_OK) {
+        if (nh.mem) (void)hipFree(nh.mem);
+        return st;
+    }
+    // the running system: step settings, the thermo ring, the box bits, chain and barostat
+    // (whose list reference is void until the list is rebuilt in the new grid)
+    nh.dev.dt = h.dev.dt;
+    nh.dev.dt2 = h.dev.dt2;
+    nh.dev.nhc_on = h.dev.nhc_on;
+    nh.dev.npt_on = h.dev.npt_on;
+    nh.thermo = h.thermo;
+    nh.thermo_cap = h.thermo_cap;
+    nh.dev.thermo = h.thermo;
+    nh.dev.thermo_cap = h.thermo_cap;
+    npt[NPT_LIST_OK] = 0.0;
+    PIS_HIP(hipMemcpy(nh.dev.box, box, sizeof(box), hipMemcpyHostToDevice));
+    PIS_HIP(hipMemcpy(nh.dev.npt, npt, sizeof(npt), hipMemcpyHostToDevice));
+    PIS_HIP(hipMemcpy(nh.dev.nhc, chain, sizeof(chain), hipMemcpyHostToDevice));
+    (void)hipFree(h.mem);
+    h = std::move(nh);
+    PIS_TRY(relayout(c));
+    return sys_set_state(c, k, pos.data(), vel.data(), frc.data());
+}
+
 // nsteps of verlet_step_nve (nvt = false) or verlet_step_nvt_nhc (nvt = true, chain set
 // by pis_nvt_init) for every system, state resident on the device.
 static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_thermo* out, bool nvt = false,
@@ -1056,40 +1252,75 @@ static pis_status batch_run(pis_ctx* c, const double* dts, uint64_t nsteps, pis_
         c->sys[k].dev.nhc_on = nvt ? 1 : 0;
         c->sys[k].dev.npt_on = npt ? 1 : 0;
     }
-    const int cap = (int)std::min<uint64_t>(nsteps, (uint64_t)1 << 30);
-    PIS_TRY(ensure_thermo(c, out ? cap : 1));
+    // The thermo ring holds at least kThermoFloor records, so calls of different lengths
+    // reuse one allocation (a reallocation would synchronise the device inside the call);
+    // a call that records more than kThermoChunk steps runs in chunks of that many, each
+    // downloaded before the next starts.
+    // NPT runs in chunks of kNptChunk steps: a system whose box would become thinner than
+    // its cell grid freezes on the device, and the host re-plans it between chunks.
+    constexpr uint64_t kThermoFloor = 1024, kThermoChunk = 8192, kNptChunk = 64;
+    uint64_t chunk = out ? std::min(nsteps, kThermoChunk) : nsteps;
+    if (npt) chunk = std::min(chunk, kNptChunk);
+    PIS_TRY(ensure_thermo(c, out ? (int)std::max(chunk, kThermoFloor) : 1));
     PIS_TRY(upload_sys(c));
     const int ns = (int)c->sys.size();
-    launch_set_ctl(c->stream, c->d_sys, ns, CTL_STEP, 0);
-    launch_set_ctl(c->stream, c->d_sys, ns, CTL_LEFT, -1);
     if (nvt) {
         if (ramp_total == 0) ramp_total = nsteps;
         if (ramp_step0 + nsteps > ramp_total)
             return fail(PIS_ERR_INVALID_ARGUMENT, "the steps run past the end of the temperature ramp");
         launch_nhc_run(c->stream, c->d_sys, ns, ramp_step0, ramp_total);
     }
-    if (npt) {
-        // verlet_step_npt_mtk (potential.rs:112-135): barostat and box update, the NVT step
-        // on the scaled state, the second scaling and momentum update; direct launches, step
-        // by step.  The tensor sums a step starts from are those the previous step ended
-        // with (nothing moves in between), so only a call's first step computes them.
-        const int rec = out != nullptr ? 1 : 0;
-        for (uint64_t t = 0; t < nsteps; ++t) {
-            if (t == 0) launch_npt(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work, 0, rec);
-            launch_npt(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work, 1, rec);
-            PIS_TRY(enqueue_one(c, t == 0, out != nullptr, true, false, true));
-            launch_npt(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work, 2, rec);
-            if ((t & 63) == 0) PIS_HIP(hipGetLastError());
+    bool resumed = false;  // the chunk starts on a re-planned system
+    for (uint64_t done = 0; done < nsteps;) {
+        const uint64_t k = std::min(chunk, nsteps - done);
+        launch_set_ctl(c->stream, c->d_sys, ns, CTL_LEFT, -1);
+        launch_set_ctl(c->stream, c->d_sys, ns, CTL_STEP, 0);
+        if (npt) {
+            // verlet_step_npt_mtk (potential.rs:112-135): barostat and box update, the NVT
+            // step on the scaled state, the second scaling and momentum update; direct
+            // launches, step by step.  The tensor sums a step starts from are those the
+            // previous step ended with (nothing moves in between), so only a call's first
+            // step, and the first after a re-plan, computes them.
+            const int rec = out != nullptr ? 1 : 0;
+            for (uint64_t t = 0; t < k; ++t) {
+                if (done + t == 0 || (resumed && t == 0))
+                    launch_npt(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work, 0, rec);
+                launch_npt(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work, 1, rec);
+                PIS_TRY(enqueue_one(c, t == 0, out != nullptr, true, false, true));
+                launch_npt(c->stream, c->d_sys, ns, c->d_atom_work, c->n_atom_work, 2, rec);
+                if ((t & 63) == 0) PIS_HIP(hipGetLastError());
+            }
+        } else {
+            PIS_TRY(enqueue_steps(c, k, out != nullptr, nvt));
         }
-    } else {
-        PIS_TRY(enqueue_steps(c, nsteps, out != nullptr, nvt));
+        PIS_HIP(hipGetLastError());
+        PIS_HIP(hipStreamSynchronize(c->stream));
+        PIS_TRY(collect_timing(c));
+        if (npt) {
+            int32_t ctl[CTL_COUNT];
+            PIS_TRY(read_ctl(c, 0, ctl));
+            if (ns == 1 && ctl[CTL_BOX]) {
+                // the steps before the freeze are done; the one that froze is run again
+                double chain[NHC_COUNT];
+                PIS_HIP(hipMemcpyAsync(chain, c->sys[0].dev.nhc, sizeof(chain), hipMemcpyDeviceToHost, c->stream));
+                PIS_HIP(hipStreamSynchronize(c->stream));
+                const uint64_t completed = (uint64_t)chain[NHC_STEP] - (ramp_step0 + done);
+                if (completed == 0 && resumed)
+                    return fail(PIS_ERR_CAPACITY, "the box shrinks faster than its cell grid can be re-planned");
+                if (out && completed) PIS_TRY(read_thermo(c, completed, out + done * (uint64_t)ns));
+                c->steps += completed;
+                done += completed;
+                PIS_TRY(replan_for_box(c, 0));
+                resumed = true;
+                continue;
+            }
+        }
+        c->steps += k;
+        PIS_TRY(check_flags(c));
+        if (out) PIS_TRY(read_thermo(c, k, out + done * (uint64_t)ns));
+        done += k;
+        resumed = false;
     }
-    PIS_HIP(hipGetLastError());
-    PIS_HIP(hipStreamSynchronize(c->stream));
-    c->steps += nsteps;
-    PIS_TRY(collect_timing(c));
-    PIS_TRY(check_flags(c));
-    if (out) PIS_TRY(read_thermo(c, nsteps, out));
     return PIS_OK;
 }
 
@@ -1136,14 +1367,13 @@ static pis_status npt_init(pis_ctx* c, const pis_nhc_params* chain, const pis_mt
     if (mtk->reserved[0] != 0.0 || mtk->reserved[1] != 0.0)
         return fail(PIS_ERR_INVALID_ARGUMENT, "pis_mtk_params.reserved must be 0");
     PIS_TRY(nvt_init(c, chain));
-    bool regraph = false;
+    bool relist = false;
     for (auto& h : c->sys) {
         // the box becomes general with the first step: raw staging with the general-box
         // image, bands for positions up to 1.25x the current extent (the box may grow)
         Plan& p = h.plan;
         double ext = 0.0;
         for (int r = 0; r < 3; ++r) ext = std::max(ext, std::fabs(p.h[r]) + std::fabs(p.h[3 + r]) + std::fabs(p.h[6 + r]));
-        if (!p.tri) regraph = true;
         p.tri = true;
         p.qmax = std::max(p.qmax, 1.25 * ext);
         for (int k = 0; k < 3; ++k)  // tile-relative staging of a general box (see make_plan)
@@ -1151,6 +1381,8 @@ static pis_status npt_init(pis_ctx* c, const pis_nhc_params* chain, const pis_mt
         set_bands(&p);
         h.dev.tri = 1;
         h.dev.mode = p.mode;
+        if (h.dev.list_shift != plan_list_shift(p)) relist = true;
+        h.dev.list_shift = plan_list_shift(p);
         h.dev.wslack = p.wslack;
         PIS_HIP(hipMemcpyAsync((void*)h.dev.pp, p.pp.data(), p.pp.size() * sizeof(PairParam), hipMemcpyHostToDevice,
                                c->stream));
@@ -1162,12 +1394,16 @@ static pis_status npt_init(pis_ctx* c, const pis_nhc_params* chain, const pis_mt
         PIS_HIP(hipMemcpyAsync(h.dev.npt, st, sizeof(st), hipMemcpyHostToDevice, c->stream));
     }
     PIS_HIP(hipStreamSynchronize(c->stream));
-    if (regraph)  // captured step graphs hold the orthorhombic kernel instantiations
-        for (auto& g : c->graph) {
-            if (g) PIS_HIP(hipGraphExecDestroy(g));
-            g = nullptr;
-        }
+    // the systems' kinds changed (general boxes): regroup the tiles, which also drops the
+    // captured step graphs holding the orthorhombic kernel instantiations
+    PIS_TRY(group_tiles(c));
     PIS_TRY(upload_sys(c));
+    if (relist) {
+        // the lists hold slot offsets for the previous staging: the next step or force
+        // call rebuilds them
+        launch_set_ctl(c->stream, c->d_sys, (int)c->sys.size(), CTL_REBUILD, 1);
+        PIS_HIP(hipStreamSynchronize(c->stream));
+    }
     c->npt_ready = true;
     return PIS_OK;
 }
@@ -1456,6 +1692,34 @@ pis_status pis_get_neighbor_list(pis_ctx* ctx, uint64_t* offsets, uint32_t* idx,
                 const uint64_t o = offsets[i] + k;
                 if (o < cap) idx[o] = buf[i * mnb + k];
             }
+    return PIS_OK;
+}
+
+pis_status pis_get_list_digest(pis_ctx* ctx, uint32_t* counts, uint64_t* hashes)
+{
+    if (!ctx) return fail(PIS_ERR_INVALID_ARGUMENT, "null context");
+    if (ctx->sys.size() != 1) return fail(PIS_ERR_INVALID_ARGUMENT, "single-system contexts only");
+    if (!counts || !hashes) return fail(PIS_ERR_INVALID_ARGUMENT, "null output");
+    int32_t ctl[CTL_COUNT];
+    PIS_TRY(read_ctl(ctx, 0, ctl));
+    ctl[CTL_REBUILD] = 1;
+    PIS_TRY(write_ctl(ctx, 0, ctl));
+    PIS_TRY(run_rebuild(ctx));
+    PIS_HIP(hipStreamSynchronize(ctx->stream));
+    PIS_TRY(check_flags(ctx));
+    const size_t n = (size_t)ctx->sys[0].plan.n;
+    void* d_out = nullptr;
+    PIS_HIP(hipMalloc(&d_out, n * 12));
+    uint64_t* d_hash = (uint64_t*)d_out;
+    uint32_t* d_cnt = (uint32_t*)(d_hash + n);
+    for (const auto& k : ctx->classes)
+        launch_list_digest(ctx->stream, ctx->d_sys, ctx->d_tile_work + k.first, k.count, k.scap, d_cnt, d_hash);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(hashes, d_hash, n * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(counts, d_cnt, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return fail(PIS_ERR_HIP, std::string("list digest: ") + hipGetErrorString(e));
     return PIS_OK;
 }
 

@@ -37,6 +37,10 @@ void launch_npt(hipStream_t st, const SysDev* d_sys, int nsys, const WorkItem* a
 void launch_list_to_orig(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work,
                          int n_tile_work, int scap, uint32_t* out, uint32_t* counts);
 
+// Per atom (original index): list length and sum(mix64(original index)) of its entries.
+void launch_list_digest(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work, int n_tile_work,
+                        int scap, uint32_t* counts, uint64_t* hashes);
+
 void launch_copy_calibration(hipStream_t st, const void* src, void* dst, size_t bytes);
 
 }  // namespace pis

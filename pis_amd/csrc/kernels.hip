@@ -45,7 +45,7 @@
 #define PIS_BUILD_SHIFTIN 1
 #endif
 #ifndef PIS_SORT_GRID_CAP
-#define PIS_SORT_GRID_CAP 1024
+#define PIS_SORT_GRID_CAP 16384
 #endif
 #ifndef PIS_BUILD_GRID_CAP
 #define PIS_BUILD_GRID_CAP (1 << 30)
@@ -318,10 +318,13 @@ constexpr double kPadFar = 1e30;  // squared: 1e60, finite in FP64, +inf in FP32
 // with raw positions (a per-pair minimum image would fold it back) it sits at the origin
 // and the force loop masks those entries by count.
 // Returns the number of staged atoms (uniform over the block); > SCAP means overflow
-// and nothing was staged.
+// and nothing was staged.  smap (SCAP + 1 bytes, may be null) receives the halo cell of
+// every slot, each cell's thread filling its run, so a slot finds its atom with one LDS
+// read instead of a binary search over the halo's cell offsets.
 template <int SCAP, int WHAT, typename T>
 __device__ int stage_tile(const SysDev& s, const State& st, const TileGeom& g, double cx, double cy,
-                          double cz, T* Px, T* Py, T* Pz, uint8_t* Pt, int* pref, int* ecell, int* stmp)
+                          double cz, T* Px, T* Py, T* Pz, uint8_t* Pt, int* pref, int* ecell, int* stmp,
+                          uint8_t* smap = nullptr)
 {
     const int tid = threadIdx.x;
     int cnt = 0;
@@ -348,14 +351,25 @@ __device__ int stage_tile(const SysDev& s, const State& st, const TileGeom& g, d
     if (tid == 0) pref[ne] = total;
     __syncthreads();
     if (total > SCAP) return total;
+    const bool mapped = smap != nullptr && ne > 1;
+    if (mapped) {
+        if (tid < ne)
+            for (int sl = ex; sl < ex + cnt; ++sl) smap[sl] = (uint8_t)tid;
+        __syncthreads();
+    }
     for (int sl = tid; sl < total; sl += blockDim.x) {
-        int lo = 0, hi = ne - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (pref[mid] <= sl)
-                lo = mid;
-            else
-                hi = mid - 1;
+        int lo = 0;
+        if (mapped) {
+            lo = smap[sl];
+        } else {
+            int hi = ne - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (pref[mid] <= sl)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
         }
         const int a = ecell[lo] + (sl - pref[lo]);
         if (WHAT == STAGE_INDEX) {
@@ -596,7 +610,16 @@ __global__ __launch_bounds__(1024) void k_scan(const SysDev* __restrict__ sys)
     const int per = ((n + 4 * (int)blockDim.x - 1) / (4 * (int)blockDim.x)) * 4;
     const int i0 = (int)threadIdx.x * per, i1 = min(i0 + per, n);
     int sum = 0;
-    for (int i = i0; i < i1; i += 4) {
+    int i = i0;
+    // four int4 loads in flight per thread (the loop is latency-bound otherwise)
+    for (; i + 16 <= i1; i += 16) {
+        int4 c[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) c[u] = *reinterpret_cast<const int4*>(s.cell_count + i + 4 * u);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) sum += (c[u].x + c[u].y) + (c[u].z + c[u].w);
+    }
+    for (; i < i1; i += 4) {
         if (i + 4 <= i1) {
             const int4 c = *reinterpret_cast<const int4*>(s.cell_count + i);
             sum += (c.x + c.y) + (c.z + c.w);
@@ -852,6 +875,7 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
     const double gym = fmax(0.0, (qy - ylo) - slack), gyp = fmax(0.0, (ylo + s.cellw[1] - qy) - slack);
     const double gzm = fmax(0.0, (qz - zlo) - slack), gzp = fmax(0.0, (zlo + s.cellw[2] - qz) - slack);
     uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;  // 8 list entries of 16 bits, w0 oldest
+    const uint32_t lsh = (uint32_t)s.list_shift;
     int cnt = 0;
     const int maxg = s.maxnb >> 3;
     const size_t npad = s.npad;
@@ -993,10 +1017,11 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
                     w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
                     w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
                     w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
-                    // entry: LDS byte offset of the slot's doubles (a multiple of 8), with the
-                    // partner's type index in the three low bits (mixtures)
+                    // entry: LDS byte offset of the slot's coordinate in the force kernel's
+                    // staging (slot << list_shift: 8-byte FP64 or 4-byte FP32 slots), with
+                    // the partner's type index in the low bits (mixtures)
                     const uint32_t sl = base + b;
-                    w3 = __builtin_amdgcn_alignbit((sl << 3) | (MULTI ? (uint32_t)Pt[sl] : 0u), w3, 16);
+                    w3 = __builtin_amdgcn_alignbit((sl << lsh) | (MULTI ? (uint32_t)Pt[sl] : 0u), w3, 16);
                     ++cnt;
                     if ((cnt & 7) == 0 && (cnt >> 3) <= maxg)
                         out[(size_t)((cnt >> 3) - 1) * npad] = u32x4{w0, w1, w2, w3};
@@ -1011,7 +1036,7 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
             w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
             w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
             w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
-            w3 = __builtin_amdgcn_alignbit((uint32_t)(SCAP << 3), w3, 16);
+            w3 = __builtin_amdgcn_alignbit((uint32_t)SCAP << lsh, w3, 16);
             ++cnt;
         }
         if ((cnt >> 3) <= maxg) out[(size_t)((cnt >> 3) - 1) * npad] = u32x4{w0, w1, w2, w3};
@@ -1020,15 +1045,27 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
     return n_real;
 }
 
-template <int SCAP, bool TRI>
+// The kind of a system's tiles (KIND_*): box shape and staging mode.
+__device__ __forceinline__ int system_kind(const SysDev& s)
+{
+    const bool rel = s.mode == MODE_TILE_REL;
+    return !s.tri ? (rel ? KIND_ORTHO : KIND_ORTHO_RAW) : (rel ? KIND_TRI : KIND_TRI_RAW);
+}
+
+// One instantiation per kind (KIND_*), so the common tile-relative kinds carry neither the
+// per-pair image of raw staging nor its registers.
+template <int SCAP, int KIND>
 __global__ __launch_bounds__(kTileThreads) void k_build_list(const SysDev* __restrict__ sys,
                                                               const WorkItem* __restrict__ tiles,
                                                               int ntw)
 {
+    constexpr bool TRI = KIND == KIND_TRI || KIND == KIND_TRI_RAW;
+    constexpr bool REL = KIND == KIND_ORTHO || KIND == KIND_TRI;
     // FP32 staging: half the LDS of the force kernel's, so more tiles share a CU.  The 64
     // slots past the padding slot take the candidate reads that run past a window's end.
     __shared__ float Px[SCAP + 1 + 64], Py[SCAP + 1 + 64], Pz[SCAP + 1 + 64];
     __shared__ uint8_t Pt[SCAP + 1 + 64];
+    __shared__ uint8_t smap[SCAP + 1];
     __shared__ int pref[kMaxHalo + 1];
     __shared__ int ecell[kMaxHalo];
     __shared__ int stmp[16];
@@ -1039,19 +1076,15 @@ __global__ __launch_bounds__(kTileThreads) void k_build_list(const SysDev* __res
         const WorkItem w = tiles[xcd_remap(b, ntw)];
         const SysDev& s = sys[w.sys];
         if (!s.ctl[CTL_REBUILD]) continue;
-        if ((s.tri != 0) != TRI) continue;  // the other instantiation builds this tile
+        if (system_kind(s) != KIND) continue;  // another instantiation builds this tile
         const State st = s.st[__builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR]) ^ 1];
         const TileGeom g = tile_geom(s, w.idx);
         const int nt = s.ntypes;
-        const bool rel = s.mode == MODE_TILE_REL;
         double cx, cy, cz;
         tile_center(s, g, cx, cy, cz);
         for (int q = threadIdx.x; q < nt * nt; q += blockDim.x) ppl[q] = s.pp[q];
-        int total;
-        if (rel)
-            total = stage_tile<SCAP, STAGE_REL>(s, st, g, cx, cy, cz, Px, Py, Pz, Pt, pref, ecell, stmp);
-        else
-            total = stage_tile<SCAP, STAGE_RAW>(s, st, g, cx, cy, cz, Px, Py, Pz, Pt, pref, ecell, stmp);
+        const int total = stage_tile<SCAP, REL ? STAGE_REL : STAGE_RAW>(s, st, g, cx, cy, cz, Px, Py, Pz, Pt, pref,
+                                                                          ecell, stmp, smap);
         if (threadIdx.x == 0) atomicMax(&s.ctl[CTL_MAXSTAGE], total);
         if (total > SCAP) {
             if (threadIdx.x == 0) atomicOr(&s.ctl[CTL_OVF_STAGE], 1);
@@ -1065,19 +1098,12 @@ __global__ __launch_bounds__(kTileThreads) void k_build_list(const SysDev* __res
         for (int base = a0 + (int)(threadIdx.x & ~63); base < a0 + span; base += blockDim.x) {
             const int a = base + lane;
             const bool active = a < a1;
-            int n;
-            if (TRI && rel)  // tile-relative staging of a general box
-                n = nt == 1 ? build_atom<SCAP, false, false, true>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz)
-                            : build_atom<SCAP, false, true, true>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz);
-            else if (TRI)  // raw positions, per-pair image of the general box
-                n = nt == 1 ? build_atom<SCAP, true, false, true>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz)
-                            : build_atom<SCAP, true, true, true>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz);
-            else if (nt == 1)
-                n = rel ? build_atom<SCAP, false, false, false>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz)
-                        : build_atom<SCAP, true, false, false>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz);
-            else
-                n = rel ? build_atom<SCAP, false, true, false>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz)
-                        : build_atom<SCAP, true, true, false>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl, total, active, a, cx, cy, cz);
+            // REL: tile-relative staging (plain differences); otherwise raw positions and the
+            // per-pair image of the box
+            const int n = nt == 1 ? build_atom<SCAP, !REL, false, TRI>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl,
+                                                                       total, active, a, cx, cy, cz)
+                                  : build_atom<SCAP, !REL, true, TRI>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl,
+                                                                      total, active, a, cx, cy, cz);
             mymax = max(mymax, n);
         }
         mymax = wave_max(mymax);
@@ -1097,10 +1123,14 @@ struct Acc {
 };
 
 // LennardJones::compute_potential (lennard_jones.rs:32-55) for one candidate, without a
-// branch: `in` selects whether the pair contributes.  Operands are finite for every
-// candidate the list can hold (the padding slot is finite), so the unselected lanes only
-// cost time.  FP64: a refined hardware reciprocal for 1/r2.  MIXED: the LJ arithmetic is
-// FP32 (r2 and the cutoff decision stay FP64) and the results accumulate in FP64.
+// branch: `in` selects whether the pair contributes.  FP64: a refined hardware reciprocal
+// for 1/r2; a candidate outside the cutoff computes with its own finite r2 and is dropped
+// by the selects (raw staging masks padding entries by count, tile-relative staging puts
+// the padding slot kPadFar away, so its r2 is finite in FP64 and fails the cutoff).  MIXED
+// with FP64 staging (raw staging only): r2 and the cutoff decision stay FP64, the LJ
+// arithmetic is FP32, and a candidate outside the cutoff enters it as r2 = +inf, which
+// v_rcp_f32 turns into +0 (force factor -0; the energy shift is masked to 0).  Results
+// accumulate in FP64.
 template <bool MIXED>
 __device__ __forceinline__ void lj_accum(const PairParam& p, double dx, double dy, double dz, double r2,
                                          bool in, Acc& acc)
@@ -1119,9 +1149,8 @@ __device__ __forceinline__ void lj_accum(const PairParam& p, double dx, double d
         acc.pe += u;
         acc.wp = fma(fp, r2, acc.wp);
     } else {
-        // a pair outside the cutoff enters as r^2 = +inf: rcp gives 0, so the force factor
-        // is -0 and, with the shift masked, so is the energy (one 32-bit select each,
-        // ahead of the FP64 conversions, instead of two per 64-bit result)
+        // one 32-bit select each for r2 and the shift, ahead of the FP64 conversions,
+        // instead of two per 64-bit result
         const float r2f = in ? (float)r2 : __builtin_inff();
         const float inv = __builtin_amdgcn_rcpf(r2f);
         const float sr2 = p.sig2f * inv;
@@ -1230,24 +1259,132 @@ __device__ __forceinline__ void atom_forces(const SysDev& s, const double* Px, c
     }
 }
 
-// KIND: KIND_ORTHO, KIND_TRI (tile-relative general box) or KIND_TRI_RAW.
-template <int SCAP, bool MIXED, int KIND>
-__global__ __launch_bounds__(kTileThreads, KIND == KIND_TRI_RAW ? 1 : (MIXED ? PIS_FORCE_MIN_WAVES_MIXED : PIS_FORCE_MIN_WAVES_FP64)) void k_force(const SysDev* __restrict__ sys,
-                                                         const WorkItem* __restrict__ tiles, int ntw,
-                                                         int integrate)
+// FP32 parameters of one type pair for the mixed pair loop with FP32 staging (32 bytes:
+// two LDS reads per list entry in mixtures).
+struct PairF {
+    float r2, sig2, c4, c24, c48, ucut, pad0, pad1;
+};
+
+__device__ __forceinline__ PairF pair_f32(const PairParam& p)
 {
-    __shared__ double Px[SCAP + 1], Py[SCAP + 1], Pz[SCAP + 1];
-    __shared__ int pref[kMaxHalo + 1];
-    __shared__ int ecell[kMaxHalo];
-    __shared__ int stmp[16];
-    __shared__ double rtmp[4 * (kTileThreads / 64)];
-    __shared__ PairParam ppl[kMaxTypes * kMaxTypes];
+    return PairF{p.r2f, p.sig2f, p.c4f, p.c24f, p.c48f, p.ucutf, 0.0f, 0.0f};
+}
+
+// The mixed pair loop over one atom's list with FP32 tile-relative staging (SoA Px, Py, Pz
+// of SCAP + 1 floats, contiguous): displacement, r^2, cutoff and LJ arithmetic in FP32
+// (LennardJones::compute_potential, lennard_jones.rs:32-55, in FP32), the 8 entries of a
+// list group summed in FP32 and each group's sums added into the FP64 accumulators.
+// Entries are byte offsets of a float slot with the partner type in the two low bits; the
+// padding slot lies kPadFar away, so padding entries have r^2 = +inf and fail the cutoff.
+// A candidate outside the cutoff enters the reciprocal as +inf (v_rcp_f32 gives +0), so
+// its force factor and virial term are -0; the energy shift is masked.
+template <int SCAP, bool MULTI>
+__device__ __forceinline__ void atom_forces_f32(const SysDev& s, const float* Px, const PairF* prow,
+                                                const PairF& p0, int a, float qx, float qy, float qz,
+                                                Acc& acc)
+{
+    constexpr uint32_t YO = (SCAP + 1) * 4, ZO = 2 * (SCAP + 1) * 4;
+    const int nn = gbl(s.nnb)[a];
+    const int ng = (nn + 7) >> 3;
+    const gu32x4* L = (const gu32x4*)(s.nbr + a);
+    const char* bx = reinterpret_cast<const char*>(Px);
+    u32x4 un = {0u, 0u, 0u, 0u};
+    if (ng > 0) un = L[0];
+    for (int gi = 0; gi < ng; ++gi) {
+        const u32x4 u = un;
+        if (gi + 1 < ng) un = L[(size_t)(gi + 1) * s.npad];
+        const uint32_t wv[4] = {u.x, u.y, u.z, u.w};
+        uint32_t off[8];
+        float X[8], Y[8], Z[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            off[e] = (wv[e >> 1] >> ((e & 1) * 16)) & 0xffffu;
+            const uint32_t ad = MULTI ? off[e] & ~3u : off[e];
+            X[e] = *reinterpret_cast<const float*>(bx + ad);
+            Y[e] = *reinterpret_cast<const float*>(bx + ad + YO);
+            Z[e] = *reinterpret_cast<const float*>(bx + ad + ZO);
+        }
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f, pe = 0.0f, wp = 0.0f;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const PairF p = MULTI ? prow[off[e] & 3u] : p0;
+            const float dx = X[e] - qx, dy = Y[e] - qy, dz = Z[e] - qz;
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            const bool in = r2 <= p.r2;
+            const float inv = __builtin_amdgcn_rcpf(in ? r2 : __builtin_inff());
+            const float sr2 = p.sig2 * inv;
+            const float sr6 = sr2 * sr2 * sr2;
+            const float w = fmaf(p.c48, sr6, -p.c24) * sr6;  // r . F_ij: the pair virial term
+            const float fp = w * inv;
+            const float uc = in ? p.ucut : 0.0f;
+            const float uu = fmaf(fmaf(p.c4, sr6, -p.c4), sr6, -uc);
+            fx = fmaf(-fp, dx, fx);
+            fy = fmaf(-fp, dy, fy);
+            fz = fmaf(-fp, dz, fz);
+            pe += uu;
+            wp += w;
+        }
+        acc.fx += (double)fx;
+        acc.fy += (double)fy;
+        acc.fz += (double)fz;
+        acc.pe += (double)pe;
+        acc.wp += (double)wp;
+    }
+}
+
+// LDS of a force-kernel workgroup; T is the staged coordinate type.
+template <typename T, int SCAP>
+struct ForceLds {
+    T Px[SCAP + 1], Py[SCAP + 1], Pz[SCAP + 1];
+    int pref[kMaxHalo + 1];
+    int ecell[kMaxHalo];
+    int stmp[16];
+    double rtmp[4 * (kTileThreads / 64)];
+    PairParam ppl[kMaxTypes * kMaxTypes];
+    PairF pf[kMaxTypes * kMaxTypes];
+    uint8_t smap[SCAP + 1];
+};
+
+template <bool F32>
+struct StageType {
+    typedef double type;
+};
+template <>
+struct StageType<true> {
+    typedef float type;
+};
+
+
+// Registers per lane a force instantiation must fit in (the launch-bounds occupancy target):
+// the FP32-staged mixed kernel aims at PIS_FORCE_MIN_WAVES_F32 waves per SIMD.
+#ifndef PIS_FORCE_MIN_WAVES_F32
+#define PIS_FORCE_MIN_WAVES_F32 5
+#endif
+template <bool MIXED, int KIND>
+struct ForceWaves {
+    static constexpr bool F32 = MIXED && (KIND == KIND_ORTHO || KIND == KIND_TRI);
+    static constexpr int value = KIND == KIND_TRI_RAW ? 1
+                                 : F32                ? PIS_FORCE_MIN_WAVES_F32
+                                 : MIXED              ? PIS_FORCE_MIN_WAVES_MIXED
+                                                      : PIS_FORCE_MIN_WAVES_FP64;
+};
+
+// KIND: one of KIND_ORTHO, KIND_TRI (tile-relative staging), KIND_ORTHO_RAW, KIND_TRI_RAW
+// (raw staging with a per-pair image).  Mixed precision with tile-relative staging stages
+// FP32 coordinates (atom_forces_f32); every other combination stages FP64.
+template <int SCAP, bool MIXED, int KIND>
+__global__ __launch_bounds__(kTileThreads, (ForceWaves<MIXED, KIND>::value)) void k_force(
+    const SysDev* __restrict__ sys, const WorkItem* __restrict__ tiles, int ntw, int integrate)
+{
+    constexpr bool REL = KIND == KIND_ORTHO || KIND == KIND_TRI;
+    constexpr bool TRI = KIND == KIND_TRI || KIND == KIND_TRI_RAW;
+    constexpr bool F32 = MIXED && REL;
+    typedef typename StageType<F32>::type T;
+    __shared__ ForceLds<T, SCAP> lds;
 
     const WorkItem w = tiles[xcd_remap(blockIdx.x, ntw)];
     const SysDev& s = sys[w.sys];
-    constexpr bool TRI = KIND != KIND_ORTHO;
-    const int kind = !s.tri ? KIND_ORTHO : (s.mode == MODE_TILE_REL ? KIND_TRI : KIND_TRI_RAW);
-    if (kind != KIND) return;                        // another instantiation takes this tile
+    if (system_kind(s) != KIND) return;             // another instantiation takes this tile
     if (integrate && s.ctl[CTL_LEFT] == 0) return;  // frozen system: the whole block leaves
     // read set: the live set, or the freshly sorted one on a rebuild step; a step writes
     // its results into the other set (wt), which becomes live at the end of the step
@@ -1264,49 +1401,44 @@ __global__ __launch_bounds__(kTileThreads, KIND == KIND_TRI_RAW ? 1 : (MIXED ? P
     const bool ahead = integrate && __builtin_amdgcn_readfirstlane(s.ctl[CTL_REMAIN]) > 1 &&
                        __builtin_amdgcn_readfirstlane(s.ctl[CTL_LEFT]) != 1 && !s.nhc_on;
     const TileGeom g = tile_geom(s, w.idx);
-    const bool rel = KIND == KIND_TRI || (KIND == KIND_ORTHO && s.mode == MODE_TILE_REL);
     double cx, cy, cz;
     tile_center(s, g, cx, cy, cz);
     const int nt = s.ntypes;
-    for (int q = threadIdx.x; q < nt * nt; q += blockDim.x) ppl[q] = s.pp[q];
-    int total;
-    if (rel)
-        total = stage_tile<SCAP, STAGE_REL>(s, st, g, cx, cy, cz, Px, Py, Pz, (uint8_t*)nullptr, pref, ecell, stmp);
-    else
-        total = stage_tile<SCAP, STAGE_RAW>(s, st, g, cx, cy, cz, Px, Py, Pz, (uint8_t*)nullptr, pref, ecell, stmp);
+    for (int q = threadIdx.x; q < nt * nt; q += blockDim.x) {
+        lds.ppl[q] = s.pp[q];
+        lds.pf[q] = pair_f32(s.pp[q]);
+    }
+    const int total = stage_tile<SCAP, REL ? STAGE_REL : STAGE_RAW, T>(
+        s, st, g, cx, cy, cz, lds.Px, lds.Py, lds.Pz, (uint8_t*)nullptr, lds.pref, lds.ecell, lds.stmp, lds.smap);
     double sum[4] = {0.0, 0.0, 0.0, 0.0};
     if (total <= SCAP) {
         const PairParam p0 = s.pp[0];  // uniform: scalar loads, no vector registers
+        const PairF p0f = pair_f32(p0);
         const int a0 = gbl(s.cell_start)[g.c0], a1 = gbl(s.cell_start)[g.c1];
         for (int a = a0 + (int)threadIdx.x; a < a1; a += blockDim.x) {
             const double xi = gbl(st.x)[a], yi = gbl(st.y)[a], zi = gbl(st.z)[a];
             const int ti = gbl(st.type)[a];
             double qx = xi, qy = yi, qz = zi;
-            if (rel) rel_to_center(s, cx, cy, cz, qx, qy, qz);
+            if (REL) rel_to_center(s, cx, cy, cz, qx, qy, qz);
             Acc acc = {0.0, 0.0, 0.0, 0.0, 0.0};
-            const ExactCtx ex = {&s, &st, pref, ecell, g.ne, xi, yi, zi};
-            if (TRI) {  // general box: tile-relative staging, or raw with a per-pair image
-                if (KIND == KIND_TRI) {
-                    if (nt == 1)
-                        atom_forces<SCAP, MIXED, false, false, true>(s, Px, Py, Pz, ppl, p0, a, ti, qx, qy, qz, ex, acc);
-                    else
-                        atom_forces<SCAP, MIXED, true, false, true>(s, Px, Py, Pz, ppl, p0, a, ti, qx, qy, qz, ex, acc);
-                } else {
-                    if (nt == 1)
-                        atom_forces<SCAP, MIXED, false, true, true>(s, Px, Py, Pz, ppl, p0, a, ti, qx, qy, qz, ex, acc);
-                    else
-                        atom_forces<SCAP, MIXED, true, true, true>(s, Px, Py, Pz, ppl, p0, a, ti, qx, qy, qz, ex, acc);
-                }
-            } else if (nt == 1) {
-                if (rel)
-                    atom_forces<SCAP, MIXED, false, false, false>(s, Px, Py, Pz, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+            if constexpr (F32) {
+                const float* Px = reinterpret_cast<const float*>(lds.Px);
+                if (nt == 1)
+                    atom_forces_f32<SCAP, false>(s, Px, lds.pf, p0f, a, (float)qx, (float)qy, (float)qz, acc);
                 else
-                    atom_forces<SCAP, MIXED, false, true, false>(s, Px, Py, Pz, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+                    atom_forces_f32<SCAP, true>(s, Px, lds.pf + ti * nt, p0f, a, (float)qx, (float)qy, (float)qz,
+                                                acc);
             } else {
-                if (rel)
-                    atom_forces<SCAP, MIXED, true, false, false>(s, Px, Py, Pz, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+                const double* Px = reinterpret_cast<const double*>(lds.Px);
+                const double* Py = reinterpret_cast<const double*>(lds.Py);
+                const double* Pz = reinterpret_cast<const double*>(lds.Pz);
+                const ExactCtx ex = {&s, &st, lds.pref, lds.ecell, g.ne, xi, yi, zi};
+                if (nt == 1)
+                    atom_forces<SCAP, MIXED, false, !REL, TRI>(s, Px, Py, Pz, lds.ppl, p0, a, ti, qx, qy, qz, ex,
+                                                               acc);
                 else
-                    atom_forces<SCAP, MIXED, true, true, false>(s, Px, Py, Pz, ppl, p0, a, ti, qx, qy, qz, ex, acc);
+                    atom_forces<SCAP, MIXED, true, !REL, TRI>(s, Px, Py, Pz, lds.ppl, p0, a, ti, qx, qy, qz, ex,
+                                                              acc);
             }
             const double m = s.mass[ti];
             // positions again from memory (L2): not held in registers across the pair loop
@@ -1386,7 +1518,7 @@ __global__ __launch_bounds__(kTileThreads, KIND == KIND_TRI_RAW ? 1 : (MIXED ? P
     } else if (threadIdx.x == 0) {
         atomicOr(&s.ctl[CTL_OVF_STAGE], 1);
     }
-    block_sum4(sum, rtmp);
+    block_sum4(sum, lds.rtmp);
     if (threadIdx.x == 0) {
         double* P = s.partial + 4 * (size_t)w.idx;
         P[0] = sum[0];
@@ -1726,8 +1858,9 @@ __global__ __launch_bounds__(kAtomThreads) void k_npt_atoms(const SysDev* __rest
 // update from the pressure tensor of the live state, the step's velocity scaling
 // exp(-dt/2 eta_dot), the box h <- exp(dt eta_dot) h with its inverse and volume, and the
 // displacement bound under which the list built in an earlier box still holds every pair
-// within the cutoff (the reference finds pairs afresh in every force call).  A box that
-// became thinner than the planned cell grid raises CTL_BOX.
+// within the cutoff (the reference finds pairs afresh in every force call).  A new box
+// thinner than the planned cell grid raises CTL_BOX and freezes the system before anything
+// of the step is applied (the host re-plans the grid and resumes from that step).
 __global__ __launch_bounds__(1024) void k_npt_begin(const SysDev* __restrict__ sys)
 {
     __shared__ double tmp[1024 + TENS_COUNT];
@@ -1738,14 +1871,33 @@ __global__ __launch_bounds__(1024) void k_npt_begin(const SysDev* __restrict__ s
     if (threadIdx.x != 0) return;
     double* npt = s.npt;
     double* box = s.box;
-    mtk_add_delta(v, npt, box[BOX_VOL], s.dt);
-    mtk_scale(npt, s.dt, -0.5, npt + NPT_SCALE);
+    // the step's barostat update on a copy, committed only when the new box still fits
+    // the planned cell grid
+    double nl[NPT_COUNT];
+    for (int k = 0; k < NPT_COUNT; ++k) nl[k] = npt[k];
+    mtk_add_delta(v, nl, box[BOX_VOL], s.dt);
+    mtk_scale(nl, s.dt, -0.5, nl + NPT_SCALE);
     double sh[9], h[9], hi[9];
-    mtk_scale(npt, s.dt, 1.0, sh);
-    for (int k = 0; k < 9; ++k) npt[NPT_HINV_OLD + k] = box[BOX_HINV + k];
+    mtk_scale(nl, s.dt, 1.0, sh);
+    for (int k = 0; k < 9; ++k) nl[NPT_HINV_OLD + k] = box[BOX_HINV + k];
     mat3_mul(sh, box + BOX_H, h);
     mat3_inverse(h, hi);
     const double vol = fabs(mat3_det(h));
+    if (s.mode != MODE_WHOLE) {
+        for (int k = 0; k < 3; ++k) {
+            const double* p = h + 3 * ((k + 1) % 3);
+            const double* q = h + 3 * ((k + 2) % 3);
+            const double cx = p[1] * q[2] - p[2] * q[1], cy = p[2] * q[0] - p[0] * q[2], cz = p[0] * q[1] - p[1] * q[0];
+            if (!(vol / sqrt(cx * cx + cy * cy + cz * cz) >= nl[NPT_RCL] * (double)s.nc[k])) {
+                // too thin for the cell grid: nothing of this step is applied and the system
+                // freezes (every later kernel skips it) until the host re-plans the grid
+                s.ctl[CTL_BOX] = 1;
+                s.ctl[CTL_LEFT] = 0;
+                return;
+            }
+        }
+    }
+    for (int k = 0; k < NPT_COUNT; ++k) npt[k] = nl[k];
     for (int k = 0; k < 9; ++k) {
         box[BOX_H + k] = h[k];
         box[BOX_HINV + k] = hi[k];
@@ -1770,14 +1922,6 @@ __global__ __launch_bounds__(1024) void k_npt_begin(const SysDev* __restrict__ s
         npt[NPT_SKIN2] = 0.0;
     } else {
         npt[NPT_SKIN2] = fmin(lim * lim, s.half_skin2);
-    }
-    if (s.mode != MODE_WHOLE) {
-        for (int k = 0; k < 3; ++k) {
-            const double* p = h + 3 * ((k + 1) % 3);
-            const double* q = h + 3 * ((k + 2) % 3);
-            const double cx = p[1] * q[2] - p[2] * q[1], cy = p[2] * q[0] - p[0] * q[2], cz = p[0] * q[1] - p[1] * q[0];
-            if (!(vol / sqrt(cx * cx + cy * cy + cz * cz) >= npt[NPT_RCL] * (double)s.nc[k])) s.ctl[CTL_BOX] = 1;
-        }
     }
 }
 
@@ -1960,9 +2104,53 @@ __global__ __launch_bounds__(kTileThreads) void k_list_to_orig(const SysDev* __r
             const uint4 u = s.nbr[(size_t)(k >> 3) * s.npad + a];
             const uint32_t wv[4] = {u.x, u.y, u.z, u.w};
             const uint32_t off = (wv[(k & 7) >> 1] >> (((k & 7) & 1) * 16)) & 0xffffu;
-            out[(size_t)oa * s.maxnb + k] = st.orig[Li[off >> 3]];
+            out[(size_t)oa * s.maxnb + k] = st.orig[Li[off >> s.list_shift]];
         }
         counts[oa] = (uint32_t)nn;
+    }
+}
+
+// splitmix64's output function (Steele, Lea & Flood 2014; Vigna's constants): the entry
+// hash of the list digest.
+__device__ __forceinline__ uint64_t mix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// Test support: per atom (original index) the list length and the order-independent row
+// hash sum(mix64(original index of each entry)) mod 2^64.
+template <int SCAP>
+__global__ __launch_bounds__(kTileThreads) void k_list_digest(const SysDev* __restrict__ sys,
+                                                               const WorkItem* __restrict__ tiles, int ntw,
+                                                               uint32_t* __restrict__ counts,
+                                                               uint64_t* __restrict__ hashes)
+{
+    __shared__ int Li[SCAP + 1];
+    __shared__ int pref[kMaxHalo + 1];
+    __shared__ int ecell[kMaxHalo];
+    __shared__ int stmp[16];
+    const WorkItem w = tiles[blockIdx.x];
+    const SysDev& s = sys[w.sys];
+    const State st = s.st[__builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR])];
+    const TileGeom g = tile_geom(s, w.idx);
+    const int total = stage_tile<SCAP, STAGE_INDEX, int>(s, st, g, 0.0, 0.0, 0.0, Li, nullptr, nullptr, nullptr,
+                                                         pref, ecell, stmp);
+    if (total > SCAP) return;
+    const int a0 = s.cell_start[g.c0], a1 = s.cell_start[g.c1];
+    for (int a = a0 + (int)threadIdx.x; a < a1; a += blockDim.x) {
+        const int nn = s.nnb[a];
+        uint64_t h = 0;
+        for (int k = 0; k < nn; ++k) {
+            const uint4 u = s.nbr[(size_t)(k >> 3) * s.npad + a];
+            const uint32_t wv[4] = {u.x, u.y, u.z, u.w};
+            const uint32_t off = (wv[(k & 7) >> 1] >> (((k & 7) & 1) * 16)) & 0xffffu;
+            h += mix64((uint64_t)st.orig[Li[off >> s.list_shift]]);
+        }
+        counts[st.orig[a]] = (uint32_t)nn;
+        hashes[st.orig[a]] = h;
     }
 }
 
@@ -2050,14 +2238,21 @@ void launch_build_list(hipStream_t st, const SysDev* d_sys, const WorkItem* tile
 {
     if (n_tile_work <= 0) return;
     const int gt = grid_cap(n_tile_work, PIS_BUILD_GRID_CAP);
-#define PIS_BUILD(S) \
-    hipLaunchKernelGGL((k_build_list<S, false>), dim3(gt), dim3(kTileThreads), 0, st, d_sys, tile_work, n_tile_work)
-#define PIS_BUILD_TRI(S) \
-    hipLaunchKernelGGL((k_build_list<S, true>), dim3(gt), dim3(kTileThreads), 0, st, d_sys, tile_work, n_tile_work)
-    if (kinds & KIND_ORTHO) PIS_SCAP_DISPATCH(scap, PIS_BUILD);
-    if (kinds & (KIND_TRI | KIND_TRI_RAW)) PIS_SCAP_DISPATCH(scap, PIS_BUILD_TRI);
-#undef PIS_BUILD
-#undef PIS_BUILD_TRI
+#define PIS_BUILD_K(S, K) \
+    hipLaunchKernelGGL((k_build_list<S, K>), dim3(gt), dim3(kTileThreads), 0, st, d_sys, tile_work, n_tile_work)
+#define PIS_BUILD_O(S) PIS_BUILD_K(S, KIND_ORTHO)
+#define PIS_BUILD_OR(S) PIS_BUILD_K(S, KIND_ORTHO_RAW)
+#define PIS_BUILD_T(S) PIS_BUILD_K(S, KIND_TRI)
+#define PIS_BUILD_TR(S) PIS_BUILD_K(S, KIND_TRI_RAW)
+    if (kinds & KIND_ORTHO) PIS_SCAP_DISPATCH(scap, PIS_BUILD_O);
+    if (kinds & KIND_ORTHO_RAW) PIS_SCAP_DISPATCH(scap, PIS_BUILD_OR);
+    if (kinds & KIND_TRI) PIS_SCAP_DISPATCH(scap, PIS_BUILD_T);
+    if (kinds & KIND_TRI_RAW) PIS_SCAP_DISPATCH(scap, PIS_BUILD_TR);
+#undef PIS_BUILD_K
+#undef PIS_BUILD_O
+#undef PIS_BUILD_OR
+#undef PIS_BUILD_T
+#undef PIS_BUILD_TR
 }
 
 void launch_force(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work, int n_tile_work,
@@ -2073,11 +2268,19 @@ void launch_force(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work
 #define PIS_FORCE_DT(S) PIS_FORCE_LAUNCH(S, false, KIND_TRI)
 #define PIS_FORCE_MR(S) PIS_FORCE_LAUNCH(S, true, KIND_TRI_RAW)
 #define PIS_FORCE_DR(S) PIS_FORCE_LAUNCH(S, false, KIND_TRI_RAW)
+#define PIS_FORCE_MO(S) PIS_FORCE_LAUNCH(S, true, KIND_ORTHO_RAW)
+#define PIS_FORCE_DO(S) PIS_FORCE_LAUNCH(S, false, KIND_ORTHO_RAW)
     if (kinds & KIND_ORTHO) {
         if (mixed)
             PIS_SCAP_DISPATCH(scap, PIS_FORCE_M);
         else
             PIS_SCAP_DISPATCH(scap, PIS_FORCE_D);
+    }
+    if (kinds & KIND_ORTHO_RAW) {
+        if (mixed)
+            PIS_SCAP_DISPATCH(scap, PIS_FORCE_MO);
+        else
+            PIS_SCAP_DISPATCH(scap, PIS_FORCE_DO);
     }
     if (kinds & KIND_TRI) {
         if (mixed)
@@ -2098,6 +2301,8 @@ void launch_force(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work
 #undef PIS_FORCE_DT
 #undef PIS_FORCE_MR
 #undef PIS_FORCE_DR
+#undef PIS_FORCE_MO
+#undef PIS_FORCE_DO
 }
 
 void launch_reduce(hipStream_t st, const SysDev* d_sys, int nsys, int flip, int record, int step)
@@ -2146,6 +2351,15 @@ void launch_sync_labels(hipStream_t st, const SysDev* d_sys, const WorkItem* ato
 void launch_flip(hipStream_t st, const SysDev* d_sys, int nsys)
 {
     hipLaunchKernelGGL(k_flip, dim3((nsys + 255) / 256), dim3(256), 0, st, d_sys, nsys);
+}
+
+void launch_list_digest(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work, int n_tile_work,
+                        int scap, uint32_t* counts, uint64_t* hashes)
+{
+    const dim3 g(n_tile_work), b(kTileThreads);
+#define PIS_DIG(S) hipLaunchKernelGGL(k_list_digest<S>, g, b, 0, st, d_sys, tile_work, n_tile_work, counts, hashes)
+    PIS_SCAP_DISPATCH(scap, PIS_DIG);
+#undef PIS_DIG
 }
 
 void launch_list_to_orig(hipStream_t st, const SysDev* d_sys, const WorkItem* tile_work,

@@ -104,7 +104,7 @@ pis_status admit(pis_server* sv, const std::shared_ptr<Job>& j)
     // tiles (its class lists them last, systems being grouped in index order)
     const Plan& p = c->sys[k].plan;
     for (const auto& q : c->classes) {
-        if (q.scap != p.scap) continue;
+        if (q.scap != p.scap || q.kind != plan_kind(p)) continue;
         launch_force(c->stream, c->d_sys, c->d_tile_work + q.first + (q.count - p.ntile), p.ntile, q.scap,
                      c->mixed, 0, plan_kind(p));
     }

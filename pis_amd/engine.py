@@ -252,6 +252,16 @@ class Engine:
                                           C.byref(total)))
         return offsets.astype(np.int64), idx[: int(total.value)].astype(np.int64)
 
+    def list_digest(self):
+        """(counts, hashes) per atom (original order) of the full list at the current
+        positions: row length and sum of mix64(j) mod 2^64 over the row."""
+        n = self.specs[0].n
+        counts = np.zeros(n, dtype=np.uint32)
+        hashes = np.zeros(n, dtype=np.uint64)
+        check(lib().pis_get_list_digest(self._ptr, counts.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                        hashes.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return counts, hashes
+
     def stats(self) -> dict:
         s = _capi.Stats()
         check(lib().pis_get_stats(self._ptr, C.byref(s)))

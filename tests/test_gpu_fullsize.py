@@ -5,9 +5,15 @@
   normwise) and the PE against its exactly rounded sum;
 * mixed-precision force RMS error <= 1e-5 against the same oracle forces;
 * Newton's third law: the total force vanishes to rounding;
-* the neighbour list is symmetric (j lists i exactly when i lists j) and its size equals
-  the oracle's pair count at rc + skin;
-* a short NVE run conserves energy and the mixed and FP64 runs agree.
+* the neighbour list equals the oracle's (build_neighbour_list at rc + skin) row by row:
+  per atom the length and the order-independent hash sum(mix64(j)) of the row, both
+  bit-identical (at 1M the device digest is also checked against the downloaded list,
+  which is symmetric);
+* pressure, the absolute virial sum_i r_i.F_i and the pair virial after one FP64 step
+  against the oracle to 1e-10;
+* a short NVE run conserves energy and the mixed and FP64 runs agree; over 10^4 steps
+  (after 300 of equilibration) the fitted energy drift of both precisions is below
+  1e-4 epsilon per atom.
 """
 import math
 
@@ -65,6 +71,73 @@ def test_fp64_forces_and_energy_at_full_size(system):
     rcl = max(p[4] for p in spec.pairs) + spec.skin
     nbar = 4.0 / 3.0 * math.pi * rcl ** 3 * n / spec.box.volume()
     assert abs(counts.mean() - nbar) < 0.02 * nbar
+
+
+def test_pair_set_digest_equals_oracle_at_full_size(system):
+    """build_neighbour_list (lennard_jones.rs:344-416) at rc + skin, every row: length and
+    row hash bit-identical to the oracle's (orc_list_digest)."""
+    name, spec, _, _ = system
+    o = po.OSystem.from_spec(spec)
+    c_ref, h_ref = o.list_digest(spec.skin)
+    with Engine(spec, "mixed") as eng:  # the mixed path's list (FP32 staging offsets)
+        eng.load()
+        c_m, h_m = eng.list_digest()
+    with Engine(spec, "fp64") as eng:
+        eng.load()
+        c_d, h_d = eng.list_digest()
+        if spec.n <= 1 << 20:
+            # the device digest against the downloaded list (the same rows, hashed on the host)
+            off, idx = eng.neighbor_list()
+            c_l, h_l = po.csr_digest(off, idx)
+            assert np.array_equal(c_d, c_l) and np.array_equal(h_d, h_l)
+    print(f"{name}: {int(c_ref.sum())} entries, mean row {c_ref.mean():.2f}", flush=True)
+    for c, h in ((c_d, h_d), (c_m, h_m)):
+        assert np.array_equal(c, c_ref)
+        assert np.array_equal(h, h_ref)
+
+
+def test_pressure_and_virials_after_one_step_at_full_size(system):
+    """Thermo of one FP64 NVE step against the oracle after the same step: pressure
+    (properties.rs:61-65), sum_i r_i.F_i (properties.rs:49-51) and the pair virial."""
+    name, spec, _, _ = system
+    if spec.n > 1 << 20:
+        pytest.skip("one 1M-atom snapshot covers it; the 4M oracle step takes minutes")
+    with Engine(spec, "fp64") as eng:
+        eng.load()
+        eng.compute_potential()
+        th = eng.run_nve(spec.dt, 1)[0, 0]
+    o = po.OSystem.from_spec(spec)
+    o.compute(po.MGR_OFFSET)
+    o.verlet_step_nve(spec.dt, po.MGR_OFFSET)
+    ke = o.kinetic_energy()
+    w_abs = o.virial_trace()
+    p_ref = o.pressure(ke)
+    w_pair = o.pair_virial()
+    scale_abs = float(np.abs((o.pos * o.force).sum(axis=1)).sum())
+    assert abs(th[5] - w_abs) <= 1e-10 * scale_abs
+    assert abs(th[4] - p_ref) <= 1e-10 * abs(2 * ke + scale_abs) / (3 * spec.box.volume())
+    assert abs(th[6] - w_pair) <= 1e-10 * abs(w_pair)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_nve_drift_over_ten_thousand_steps(precision):
+    """The north-star bar on the 1M-atom liquid: after 300 steps of equilibration, the
+    least-squares slope of H over 10^4 NVE steps is below 1e-4 epsilon per atom per 10^4
+    steps (Simulation::output prints H every step, simulation.rs:83-86)."""
+    spec = CONFIGS["lj_liquid_1M_rc2.5"]()
+    with Engine(spec, precision) as eng:
+        eng.load()
+        eng.compute_potential()
+        eng.run_nve(spec.dt, 300, record=False)
+        h = eng.run_nve(spec.dt, 10000)[:, 0, 2]
+    t = np.arange(h.size, dtype=np.float64)
+    slope = np.polyfit(t, h - h[0], 1)[0]
+    drift = abs(slope) * 1e4 / spec.n
+    span = abs(h[-1] - h[0]) / spec.n
+    print(f"{precision}: fitted drift {drift:.3e}, end-start {span:.3e} eps/atom per 1e4 steps", flush=True)
+    assert np.all(np.isfinite(h))
+    assert drift <= 1e-4
+    assert span <= 1e-4
 
 
 def test_mixed_force_error_at_full_size(system):

@@ -9,8 +9,9 @@
 * the neighbour list outlives box changes while the deformation-aware displacement check
   allows it: after a 300-step NPT run with few rebuilds, the forces equal the oracle's
   brute force over all pairs in the final box;
-* a barostat that squeezes the box below the planned cell grid stops with
-  PIS_ERR_CAPACITY instead of missing pairs.
+* a barostat that squeezes the box below the planned cell grid does not miss pairs: the
+  grid is planned again for the smaller box and the run goes on tracking the oracle (whose
+  cells follow the box every step, like the reference's).
 
 Tilted test systems are made by shearing an orthorhombic liquid together with its box
 (x -> D x, h -> D h), which keeps them periodic without overlaps.
@@ -19,7 +20,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from pis_amd import Engine, PisError, SimulationBox, workloads as W
+from pis_amd import Engine, SimulationBox, workloads as W
 from tests.conftest import force_err, force_rms_err
 
 pytestmark = pytest.mark.gpu
@@ -148,12 +149,41 @@ def test_npt_list_reuse_keeps_every_pair(name, precision):
         assert force_rms_err(f, o.force) <= 1e-5
 
 
-def test_npt_squeezed_below_the_cell_grid_stops_with_capacity_error():
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_npt_compression_below_the_cell_grid_replans_and_tracks_oracle(precision):
+    """A large target pressure shrinks the 2048-atom box below its planned 4 x 4 x 4 grid of
+    rc + skin cells (the oracle crosses it at step 57): the run goes on in a re-planned
+    grid, with thermo and box tracking the oracle (neighbour_list.rs:45-57 re-divides the
+    cells in every force call)."""
     spec = W.lj_liquid(8, 0.8442, 0.9, seed=66, rcut=2.5, skin=0.3, dt=0.004)
-    with Engine(spec, "fp64") as eng:
+    steps = 120
+    o = po.OSystem.from_spec(spec)
+    ref, hb = o.run_npt(spec.dt, steps, 0.9, 0.9, 0.5, 12.0, 8.0, 3, kb=spec.kb)
+    with Engine(spec, precision) as eng:
         eng.load()
         eng.compute_potential()
         eng.npt_init(0.9, 0.9, 0.5, 12.0, 8.0, 3)  # a large target pressure: the box shrinks
-        with pytest.raises(PisError) as e:
-            eng.run_npt(spec.dt, 80)  # the oracle crosses the grid at step 57
-        assert e.value.status == 10
+        th = np.concatenate([eng.run_npt(spec.dt, 50, ramp=(0, steps))[:, 0, :],
+                             eng.run_npt(spec.dt, steps - 50, ramp=(50, steps))[:, 0, :]])
+        h, _ = eng.npt_box(0)
+        pos, _, f = eng.get_state(0)
+    h_ref = hb[-1].reshape(3, 3).T
+    # the final box is narrower than the starting grid allowed: 4 cells of rc + skin
+    vol = abs(np.linalg.det(h))
+    widths = [vol / np.linalg.norm(np.cross(h[:, (k + 1) % 3], h[:, (k + 2) % 3])) for k in range(3)]
+    print(f"{precision}: final perpendicular widths {widths}", flush=True)
+    assert min(widths) < 4 * 2.8
+    tol = 1e-8 if precision == "fp64" else 1e-5
+    scale = np.abs(ref[:, 0]).max()
+    for col in (0, 1, 2):  # PE, KE, H
+        assert np.abs(th[:, col] - ref[:, col]).max() <= tol * scale, col
+    assert np.abs(h - h_ref).max() <= tol * np.abs(h_ref).max()
+    # the final forces against every pair of the final box
+    spec.box = SimulationBox(h.T.reshape(9))
+    spec.positions = pos
+    o2 = po.OSystem.from_spec(spec)
+    o2.compute(po.MGR_BRUTE)
+    if precision == "fp64":
+        assert force_err(f, o2.force) <= 1e-10
+    else:
+        assert force_rms_err(f, o2.force) <= 1e-5

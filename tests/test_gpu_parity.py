@@ -106,11 +106,71 @@ def test_small_box_pair_set_matches_brute_force(n):
     check_pe(pe, spec, pe_ref)
 
 
-def test_mixed_force_rms_error():
-    spec = W.lj_liquid(10, 0.8442, 0.72, seed=3)
-    _, f = gpu_forces(spec, "mixed")
-    _, f_ref = oracle_forces(spec)
-    assert force_rms_err(f, f_ref) <= 1e-5
+@pytest.mark.parametrize("name", sorted(SPECS))
+def test_mixed_force_rms_error(name):
+    """Mixed precision (FP32 pair arithmetic) against the FP64 oracle: force RMS error
+    <= 1e-5 relative and PE within 2e-6 relative, on every parity system (single species
+    and Kob-Andersen mixtures, tile-relative and raw staging)."""
+    spec = SPECS[name]()
+    pe, f = gpu_forces(spec, "mixed")
+    pe_ref, f_ref = oracle_forces(spec)
+    err = force_rms_err(f, f_ref)
+    print(f"{name}: mixed force RMS error {err:.3e}, PE {pe} vs {pe_ref}", flush=True)
+    assert err <= 1e-5
+    assert abs(pe - pe_ref) <= 2e-6 * abs(pe_ref)
+
+
+def test_slab_denser_than_its_box_average_replans_its_tiles():
+    """A liquid slab filling a third of its box: tiles planned for the box's average density
+    meet halos of about 4,000 atoms, more than the largest LDS staging class holds even after
+    growing; loading the state plans the tiles again for the measured peak, and forces,
+    energy and a short run match the oracle."""
+    from pis_amd import SimulationBox
+
+    spec = W.lj_liquid(12, 0.8442, 0.72, seed=31)
+    L = spec.box.lengths
+    spec.box = SimulationBox((L[0], 0.0, 0.0, 0.0, L[1], 0.0, 0.0, 0.0, 3.0 * L[2]))
+    pe_ref, f_ref = oracle_forces(spec)
+    o = po.OSystem.from_spec(spec)
+    ref = o.run_nve(spec.dt, 50, po.MGR_OFFSET, kb=spec.kb)
+    for precision in ("fp64", "mixed"):
+        with Engine(spec, precision) as eng:
+            eng.load()
+            pe = float(eng.compute_potential()[0])
+            _, _, f = eng.get_state(0)
+            th = eng.run_nve(spec.dt, 50)[:, 0, :]
+        if precision == "fp64":
+            assert force_err(f, f_ref) <= 1e-10
+            check_pe(pe, spec, pe_ref)
+            assert np.abs(th[:, 2] - ref[:, 2]).max() <= 1e-9 * np.abs(ref[:, 0]).max()
+        else:
+            assert force_rms_err(f, f_ref) <= 1e-5
+            assert np.abs(th[:, 2] - ref[:, 2]).max() <= 1e-5 * np.abs(ref[:, 0]).max()
+
+
+def test_nve_pressure_and_virials_track_oracle():
+    """Every step of a 100-step FP64 run of the C1 argon crystal: pressure
+    (properties.rs:61-65), the absolute virial sum_i r_i.F_i (properties.rs:49-51) and the
+    periodic pair virial against the oracle's own trajectory, to 1e-10."""
+    spec = _jittered_argon()
+    nsteps = 100
+    with Engine(spec, "fp64") as eng:
+        eng.load()
+        eng.compute_potential()
+        th = eng.run_nve(spec.dt, nsteps)[:, 0, :]
+    o = po.OSystem.from_spec(spec)
+    o.compute(po.MGR_OFFSET)
+    vol = spec.box.volume()
+    for k in range(nsteps):
+        o.verlet_step_nve(spec.dt, po.MGR_OFFSET)
+        ke = o.kinetic_energy()
+        w_abs = o.virial_trace()
+        scale_abs = float(np.abs((o.pos * o.force).sum(axis=1)).sum())
+        p_ref = o.pressure(ke)
+        w_pair = o.pair_virial()
+        assert abs(th[k, 5] - w_abs) <= 1e-10 * scale_abs, k
+        assert abs(th[k, 4] - p_ref) <= 1e-10 * (2 * ke + scale_abs) / (3 * vol), k
+        assert abs(th[k, 6] - w_pair) <= 1e-10 * abs(w_pair), k
 
 
 def test_argon_c1_tracks_oracle_energies():
