@@ -155,6 +155,22 @@ pis_status pis_get_state(pis_ctx* ctx, double* pos, double* vel, double* force);
  * overwritten with the computed forces (what verlet_step_nve uses as F(t)). */
 pis_status pis_compute_potential(pis_ctx* ctx, double* forces_inout, double* pe);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * forces and PE of pis_compute_potential for one system whose positions (AoS xyz, 3n
+ * doubles) and 1-based atom types (n uint32, desc->type_ids is ignored) are already in
+ * device memory.  Forces are ADDED into d_forces_inout (3n doubles, device, may be NULL);
+ * *d_pe (device, may be NULL) receives the PE.  Every buffer the kernels need beyond these
+ * lives in a static 64 MiB device array per GPU, held until the process ends: a system
+ * that needs more is rejected with PIS_ERR_CAPACITY before any device work.  Work goes on
+ * `stream` (a hipStream_t, NULL for the null stream) and the call returns once it has
+ * finished.  Calls on one GPU share the array and are serialised by a lock; it keeps the
+ * last call's contents, and each call clears what it uses.  No hipMalloc, hipFree or
+ * allocating library call is made.  The result is bit-identical to pis_ctx_create +
+ * pis_set_state + pis_compute_potential on the same input. */
+pis_status pis_compute_potential_noalloc(const pis_system_desc* desc, const double* d_pos, const uint32_t* d_types,
+                                         double* d_forces_inout, double* d_pe, void* stream);
+
 /* nsteps of PotentialManager::verlet_step_nve (potential.rs:15-33) with state resident
  * on the device.  per_step_out (nsteps records, may be NULL) receives thermo after each
  * step; its pe is what verlet_step_nve returns. */

@@ -162,6 +162,7 @@ SIGNATURES = {
     "pis_set_state": (C.c_int, [_P, _DP, _DP, _DP]),
     "pis_get_state": (C.c_int, [_P, _DP, _DP, _DP]),
     "pis_compute_potential": (C.c_int, [_P, _DP, _DP]),
+    "pis_compute_potential_noalloc": (C.c_int, [C.POINTER(SystemDesc), _P, _P, _P, _P, _P]),
     "pis_verlet_step_nve": (C.c_int, [_P, C.c_double, C.c_uint64, C.POINTER(Thermo)]),
     "pis_get_neighbor_list": (
         C.c_int,

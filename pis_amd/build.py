@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["csrc/kernels.hip", "csrc/engine.hip", "csrc/server.hip", "csrc/dump.cpp"]
+SOURCES = ["csrc/kernels.hip", "csrc/engine.hip", "csrc/server.hip", "csrc/noalloc.hip", "csrc/dump.cpp"]
 OUT = os.path.join(HERE, "libpis_hip.so")
 
 # -ffp-contract=off: the reference (Rust) never fuses a*b+c; kernels ask for fma() where

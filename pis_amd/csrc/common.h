@@ -39,7 +39,8 @@ enum {
                        // positions of the next step while more steps follow)
     CTL_REBUILD_NEXT = 11,  // skin check of those advanced positions
     CTL_BOX = 12,      // the box (NPT) became too thin for the planned cell grid
-    CTL_COUNT = 13
+    CTL_BADINPUT = 13, // device-resident input rejected: bit 0 atom type, bit 1 position
+    CTL_COUNT = 14
 };
 
 // Device-resident box of a system (SysDev::box): h and h^-1 column-major, |det h|.  The
@@ -186,6 +187,20 @@ __host__ __device__ inline int32_t sorted_cell_id(const int32_t* nc, const int32
 struct WorkItem {
     int32_t sys;
     int32_t idx;  // first atom of a 256-atom chunk, or tile id
+};
+
+// Everything the allocation-free entry point puts into its static scratch array besides
+// zeros, passed by value to one set-up kernel (no host-to-device copy): the descriptor
+// (pointers into the scratch), where it goes, the work tables and the small tables.
+struct NoAllocSetup {
+    SysDev dev;
+    SysDev* dev_slot;
+    WorkItem* atom_work;
+    WorkItem* tile_work;
+    int32_t n_atom_work, n_tile_work;
+    PairParam pp[kMaxTypes * kMaxTypes];
+    double mass[kMaxTypes];
+    double box[BOX_COUNT];
 };
 
 }  // namespace pis
