@@ -302,7 +302,7 @@ def run_dump(spec, steps, cadence):
     return out
 
 
-def run_ensemble(steps, warmup, direct_steps):
+def run_ensemble(steps, warmup, direct_steps, accuracy=False):
     """Batched ensemble: 512 Kob-Andersen systems (256-32,768 atoms, log-uniform from seed 7,
     density 1.2, T* 0.5-1.5) in one context, mixed precision.  Each step is one replay of
     the captured step graph (one launch per kernel per LDS staging class over all systems);
@@ -333,17 +333,56 @@ def run_ensemble(steps, warmup, direct_steps):
             out["direct_launch_value"] = round(natoms * direct_steps / sec, 1)
             out["direct_launch_steps"] = direct_steps
     out["unit"] = "atom-steps/s"
+    if accuracy:
+        out["accuracy_sample"] = sample_accuracy(specs, 8, 2030, 10000)
     return out
 
 
-def serving_jobs(n_jobs, seed=2028):
+def sample_accuracy(specs, sample, seed, drift_steps, warm=300):
+    """Force RMS error of mixed precision at step 0 against the oracle's full-list forces,
+    and NVE drift over `drift_steps` recorded steps after `warm`, on `sample` seeded systems
+    of a batch.  They run together in one mixed-precision context: a system's arithmetic
+    does not depend on the other systems of its launches, so these are the batch's numbers."""
+    from oracle import pyoracle as po
+    from pis_amd import Engine
+
+    pick = sorted(np.random.default_rng(seed).choice(len(specs), sample, replace=False).tolist())
+    sub = [specs[k] for k in pick]
+    dts = [s.dt for s in sub]
+    errs = []
+    with Engine(sub, "mixed") as eng:
+        eng.load()
+        eng.compute_potential()
+        for k, s in enumerate(sub):
+            f = eng.get_state(k)[2]
+            f_ref, _ = po.OSystem.from_spec(s).forces_subset(np.arange(s.n))
+            rms = math.sqrt(float((f_ref ** 2).sum(axis=1).mean()))
+            errs.append(math.sqrt(float(((f - f_ref) ** 2).sum(axis=1).mean())) / rms)
+        eng.set_graph_mode(True)
+        eng.run_nve(dts, warm, record=False)
+        th = eng.run_nve(dts, drift_steps)
+    drifts = [drift_stats(th[:, k, 2], s.n)["fit_per_1e4_steps"] for k, s in enumerate(sub)]
+    return {
+        "systems": pick,
+        "atoms": [s.n for s in sub],
+        "force_rms_err_mixed_max": max(errs),
+        "force_rms_err_mixed_median": float(np.median(errs)),
+        "drift_fit_per_1e4_steps_max": max(drifts),
+        "drift_fit_per_1e4_steps_median": float(np.median(drifts)),
+        "drift_steps": drift_steps,
+        "warmup": warm,
+        "unit_drift": "epsilon/atom",
+    }
+
+
+def serving_jobs(n_jobs, rate, seed=2028):
     """Serving stream: n_jobs seeded single-species LJ boxes, atoms log-uniform on
     [500, 200000], density U(0.7, 1.1), T* U(0.7, 1.5), steps log-uniform on [100, 5000],
-    Poisson arrivals at 50 jobs/s (exponential gaps)."""
+    Poisson arrivals at `rate` jobs/s (exponential gaps)."""
     from pis_amd import workloads as W
 
     rng = np.random.default_rng(seed)
-    gaps = rng.exponential(1.0 / 50.0, size=n_jobs)
+    gaps = rng.exponential(1.0 / rate, size=n_jobs)
     arrivals = np.cumsum(gaps) - gaps[0]
     specs = []
     for k in range(n_jobs):
@@ -357,14 +396,23 @@ def serving_jobs(n_jobs, seed=2028):
     return specs, arrivals
 
 
-def run_serving(n_jobs, budget_gib=64, round_steps=50):
+def offered_load(rate):
+    """Mean atom-steps/s the stream asks for: rate x E[atoms] x E[steps], both
+    log-uniform: E = (b - a) / ln(b / a)."""
+    e_atoms = (200000 - 500) / np.log(200000 / 500)
+    e_steps = (5000 - 100) / np.log(5000 / 100)
+    return rate * e_atoms * e_steps
+
+
+def run_serving(n_jobs, rate, budget_gib=64, round_steps=50):
     """Submits the stream at its arrival times (a feeder thread) to one mixed-precision
-    server and waits for every job; latency is completion minus submission."""
+    server and waits for every job; latency is completion minus submission.  Jobs are
+    built before the clock starts, so the feeder only sleeps and submits."""
     import threading
 
     from pis_amd import Server
 
-    specs, arrivals = serving_jobs(n_jobs)
+    specs, arrivals = serving_jobs(n_jobs, rate)
     ids = [None] * n_jobs
     with Server(hbm_budget_bytes=budget_gib << 30, max_systems=256, round_steps=round_steps,
                 precision="mixed") as srv:
@@ -386,27 +434,44 @@ def run_serving(n_jobs, budget_gib=64, round_steps=50):
     if bad:
         raise RuntimeError(f"{len(bad)} serving jobs failed: {bad[0].error}")
     lat = np.array([r.latency_s for r in res])
+    wait = np.array([r.admit_s - r.submit_s for r in res])
     first = min(r.submit_s for r in res)
     last = max(r.done_s for r in res)
+    last_arrival = max(r.submit_s for r in res)
     work = sum(r.n_atoms * r.steps for r in res)
+    # steady state: the jobs that arrived after the first 20 % and before the last 20 % of
+    # the arrival window (no ramp-up, no drain)
+    lo, hi = first + 0.2 * (last_arrival - first), first + 0.8 * (last_arrival - first)
+    mid = np.array([lo <= r.submit_s <= hi for r in res])
     return {
         "jobs": n_jobs,
-        "arrival_rate_jobs_per_s": 50,
+        "arrival_rate_jobs_per_s": rate,
+        "offered_load": round(float(offered_load(rate)), 1),
         "p50_s": round(float(np.percentile(lat, 50)), 4),
         "p99_s": round(float(np.percentile(lat, 99)), 4),
         "mean_s": round(float(lat.mean()), 4),
         "max_s": round(float(lat.max()), 4),
+        "steady_jobs": int(mid.sum()),
+        "steady_p50_s": round(float(np.percentile(lat[mid], 50)), 4),
+        "steady_p99_s": round(float(np.percentile(lat[mid], 99)), 4),
+        "queue_wait_p99_s": round(float(np.percentile(wait, 99)), 4),
         "throughput": round(work / (last - first), 1),
         "unit": "atom-steps/s",
         "makespan_s": round(last - first, 3),
+        "arrival_window_s": round(last_arrival - first, 3),
+        "offered_sampled": round(work / max(last_arrival - first, 1e-9), 1),
+        "drain_s": round(last - last_arrival, 3),
         "atom_steps": int(work),
         "peak_resident_jobs": st["peak_resident"],
         "peak_bytes": st["peak_bytes"],
+        "pool_bytes": st["pool_bytes"],
+        "graph_instantiations": st["graph_instantiations"],
+        "graph_build_s": round(st["graph_build_s"], 4),
+        "rounds": st["rounds"],
         "budget_bytes": budget_gib << 30,
         "round_steps": round_steps,
         "precision": "mixed",
     }
-
 
 def cpu_baseline(spec, steps):
     """The oracle (faithful LJVOffsetManager + verlet_step_nve, 1 core) on `steps` steps
@@ -495,8 +560,10 @@ def main():
     ap.add_argument("--dump-cadence", type=int, default=20)
     ap.add_argument("--ensemble-steps", type=int, default=5000,
                     help="steps of the 512-system batched ensemble (0: skip)")
-    ap.add_argument("--serve-jobs", type=int, default=100,
-                    help="jobs of the Poisson serving stream (0: skip)")
+    ap.add_argument("--serve-jobs", type=int, default=500,
+                    help="jobs of each Poisson serving stream (0: skip)")
+    ap.add_argument("--serve-low-rate", type=float, default=25.0,
+                    help="arrival rate (jobs/s) of the second, below-capacity serving stream")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30,
                     help="bytes of the streaming-copy calibration run after the timed runs (0: skip)")
     args = ap.parse_args()
@@ -567,16 +634,26 @@ def main():
     ensemble = None
     if args.ensemble_steps:
         log(f"batched ensemble: 512 KA systems, {args.ensemble_steps} steps, graph replay")
-        ensemble = run_ensemble(args.ensemble_steps, 100, min(args.ensemble_steps, 1000))
+        ensemble = run_ensemble(args.ensemble_steps, 100, min(args.ensemble_steps, 1000), accuracy=not args.no_cpu)
         log(f"ensemble: {ensemble['total_atoms']} atoms, {ensemble['value']:.4e} atom-steps/s "
             f"(direct launches {ensemble.get('direct_launch_value', 0):.4e})")
 
-    serving = None
+    serving = serving_low = None
     if args.serve_jobs:
-        log(f"serving: {args.serve_jobs} Poisson jobs at 50/s, continuous batching")
-        serving = run_serving(args.serve_jobs)
-        log(f"serving: p50 {serving['p50_s']} s, p99 {serving['p99_s']} s, "
-            f"{serving['throughput']:.4e} atom-steps/s over {serving['makespan_s']} s")
+        for rate in (50.0, args.serve_low_rate):
+            log(f"serving: {args.serve_jobs} Poisson jobs at {rate:g}/s "
+                f"(offered {offered_load(rate):.3e} atom-steps/s), continuous batching")
+            r = run_serving(args.serve_jobs, rate)
+            log(f"serving: p50 {r['p50_s']} s, p99 {r['p99_s']} s (steady {r['steady_p50_s']} / "
+                f"{r['steady_p99_s']} s), {r['throughput']:.4e} atom-steps/s over {r['makespan_s']} s, "
+                f"drain {r['drain_s']} s, {r['graph_instantiations']} graph builds ({r['graph_build_s']} s)")
+            if rate == 50.0:
+                serving = r
+            else:
+                serving_low = r
+        if not args.no_cpu:
+            log("serving: force error and 10^4-step drift on 8 sampled jobs")
+            serving["accuracy_sample"] = sample_accuracy(serving_jobs(args.serve_jobs, 50.0)[0], 8, 2031, 10000)
 
     copy_gbs = None
     if args.copy_bytes:
@@ -690,6 +767,7 @@ def main():
         "batched_ensemble": ensemble,
         "pcie_inclusive_per_call": per_call,
         "job_latency": serving,
+        "job_latency_below_capacity": serving_low,
     }
     res.update(out_extra)
     print(json.dumps(res), flush=True)

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define PIS_ABI_VERSION 1
+#define PIS_ABI_VERSION 2
 
 /* Status codes.  1-4 map to PisError variants (errors.rs:71-99). */
 typedef enum pis_status {
@@ -290,6 +290,10 @@ typedef struct {
     uint64_t submitted, completed, failed, rounds, atom_steps;
     uint64_t resident, pending, bytes_in_use, peak_bytes;
     uint32_t peak_resident, reserved;
+    uint64_t graph_instantiations; /* round step graphs captured and instantiated (after
+                                      every change of the resident set) */
+    double graph_build_s;          /* host seconds those captures and instantiations took */
+    uint64_t pool_bytes;           /* device memory the server's pool holds */
 } pis_server_stats;
 
 pis_status pis_server_create(uint64_t hbm_budget_bytes, uint32_t max_systems, uint32_t round_steps,

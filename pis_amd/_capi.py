@@ -144,6 +144,9 @@ class ServerStats(C.Structure):
         ("peak_bytes", C.c_uint64),
         ("peak_resident", C.c_uint32),
         ("reserved", C.c_uint32),
+        ("graph_instantiations", C.c_uint64),
+        ("graph_build_s", C.c_double),
+        ("pool_bytes", C.c_uint64),
     ]
 
 

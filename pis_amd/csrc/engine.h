@@ -113,6 +113,11 @@ struct pis_ctx {
     // captured steps, [record + 2 nvt + 4 steady]: a call's first step (with the position
     // update kernel) and its later NVE steps (positions already advanced by the force kernel)
     hipGraphExec_t graph[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    // a membership change (work tables, classes) leaves the captured steps stale: the next
+    // call captures the step again and instantiates it; the host time this takes is counted
+    bool graph_stale[8] = {false, false, false, false, false, false, false, false};
+    uint64_t graph_instantiations = 0;
+    double graph_build_s = 0.0;
     bool nhc_ready = false;  // pis_nvt_init has set every system's chain
     bool npt_ready = false;  // pis_npt_init has set every system's barostat
     // staging classes after the first run their list build and force on streams of their

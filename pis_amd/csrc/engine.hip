@@ -1,6 +1,7 @@
 // engine.hip -- host side of the MI355X LJ engine: planning, device memory, the
 // device-resident step loop and the C ABI of include/pis_hip.h.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <map>
 #include <cstdio>
@@ -649,10 +650,7 @@ pis_status pis::group_tiles(pis_ctx* c)
         }
         c->nlanes = nl;
     }
-    for (auto& g : c->graph) {
-        if (g) PIS_HIP(hipGraphExecDestroy(g));
-        g = nullptr;
-    }
+    for (auto& stale : c->graph_stale) stale = true;
     if (!c->fork && c->nlanes > 1)
         PIS_HIP(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
     while ((int)c->side.size() + 1 < c->nlanes) {
@@ -1150,7 +1148,8 @@ pis_status pis::enqueue_steps(pis_ctx* c, uint64_t nsteps, bool record, bool nvt
     if (c->use_graph && !c->timing) {
         for (int steady = 0; steady < 2; ++steady) {
             const int gi = (record ? 1 : 0) + (nvt ? 2 : 0) + 4 * steady;
-            if (c->graph[gi] || (steady && nsteps < 2)) continue;
+            if ((c->graph[gi] && !c->graph_stale[gi]) || (steady && nsteps < 2)) continue;
+            const auto tg = std::chrono::steady_clock::now();
             hipGraph_t g = nullptr;
             PIS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             const uint64_t launches_before = c->force_launches;
@@ -1162,9 +1161,14 @@ pis_status pis::enqueue_steps(pis_ctx* c, uint64_t nsteps, bool record, bool nvt
                 return cs;
             }
             if (ce != hipSuccess) return fail(PIS_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+            c->graph_stale[gi] = false;
+            if (c->graph[gi]) PIS_HIP(hipGraphExecDestroy(c->graph[gi]));
+            c->graph[gi] = nullptr;
             hipError_t e = hipGraphInstantiate(&c->graph[gi], g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             if (e != hipSuccess) return fail(PIS_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+            c->graph_instantiations++;
+            c->graph_build_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - tg).count();
         }
         const int g0 = (record ? 1 : 0) + (nvt ? 2 : 0);
         for (uint64_t t = 0; t < nsteps; ++t) PIS_HIP(hipGraphLaunch(c->graph[g0 + (t ? 4 : 0)], c->stream));

@@ -44,6 +44,9 @@
 #ifndef PIS_BUILD_SHIFTIN
 #define PIS_BUILD_SHIFTIN 1
 #endif
+#ifndef PIS_BUILD_MIN_WAVES  // waves per SIMD the list build's registers must allow
+#define PIS_BUILD_MIN_WAVES 1
+#endif
 #ifndef PIS_SORT_GRID_CAP
 #define PIS_SORT_GRID_CAP 16384
 #endif
@@ -1055,7 +1058,7 @@ __device__ __forceinline__ int system_kind(const SysDev& s)
 // One instantiation per kind (KIND_*), so the common tile-relative kinds carry neither the
 // per-pair image of raw staging nor its registers.
 template <int SCAP, int KIND>
-__global__ __launch_bounds__(kTileThreads) void k_build_list(const SysDev* __restrict__ sys,
+__global__ __launch_bounds__(kTileThreads, PIS_BUILD_MIN_WAVES) void k_build_list(const SysDev* __restrict__ sys,
                                                               const WorkItem* __restrict__ tiles,
                                                               int ntw)
 {

@@ -1,14 +1,21 @@
 // server.hip -- continuous batching of NVE simulation jobs on one GPU (pis_server_*).
 //
-// One worker thread owns a context whose systems are the resident jobs.  It loops over
-// rounds: retire the jobs whose step budget ran out (final state and thermo copied to
-// the host), admit queued jobs in arrival order while the HBM budget and the slot limit
-// allow (upload, list build, initial forces of that system alone), then advance every
-// resident job by up to `round_steps` steps in one batched step sequence per step.  A
-// job's step budget lives on the device (ctl[CTL_LEFT]): a job that finishes inside a
-// round is frozen by the kernels for the rest of it.  A system's arithmetic does not
-// depend on which other systems share its launches, so a job's result is bit-identical
-// to the same job run alone.
+// One worker thread owns three contexts, each with its own stream:
+//   * `ctx`, the running batch: every resident job's system, advanced together in rounds
+//     of `round_steps` batched steps replayed from captured graphs (a job whose step
+//     budget, ctl[CTL_LEFT], runs out inside a round is frozen by the kernels for the rest
+//     of it);
+//   * `actx`, admission: while a round runs, queued jobs that fit the HBM budget and the
+//     slot limit are placed, loaded, sorted, listed and given their initial forces here;
+//   * `rctx`, collection: jobs that finished in the last round move here, and their final
+//     state and thermo leave the device while the next round runs.
+// Between rounds, finished jobs move ctx -> rctx and admitted ones actx -> ctx.  A system's
+// device memory stays where it is; only the contexts' work tables change, and the captured
+// round's step graphs are captured again (a few hundred microseconds of host time, once per
+// round that changed the resident set).  Every system lives in one block of a device memory pool that grows by slabs and is freed only with the server,
+// so neither admission nor retirement allocates or frees device memory per job.  A
+// system's arithmetic does not depend on which other systems share its launches, so a
+// job's result is bit-identical to the same job run alone.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -31,7 +38,8 @@ struct Job {
     std::vector<double> pos, vel;
     double dt = 0.0;
     uint64_t nsteps = 0;
-    size_t bytes = 0;
+    size_t bytes = 0;     // its block of the pool: the system and round_steps thermo records
+    void* mem = nullptr;  // that block while the job is on the device
     double t_submit = 0.0, t_admit = -1.0, t_done = -1.0;
     pis_status status = PIS_OK;
     std::string error;
@@ -40,14 +48,91 @@ struct Job {
     bool done = false;
 };
 
+// Device memory in slabs of at least `slab_bytes`, handed out first-fit in 256-byte units
+// and taken back with adjacent free ranges merged.  Slabs are freed with the pool only
+// (hipFree synchronises the device).
+class DevicePool {
+public:
+    explicit DevicePool(size_t slab_bytes) : slab_bytes_(slab_bytes) {}
+    ~DevicePool()
+    {
+        for (auto& s : slabs_) (void)hipFree(s.base);
+    }
+    DevicePool(const DevicePool&) = delete;
+    DevicePool& operator=(const DevicePool&) = delete;
+
+    pis_status alloc(size_t bytes, void** out)
+    {
+        bytes = align256(bytes);
+        for (auto& s : slabs_)
+            for (auto it = s.free.begin(); it != s.free.end(); ++it)
+                if (it->second >= bytes) {
+                    const size_t off = it->first, len = it->second;
+                    s.free.erase(it);
+                    if (len > bytes) s.free[off + bytes] = len - bytes;
+                    *out = s.base + off;
+                    return PIS_OK;
+                }
+        Slab s;
+        s.size = std::max(slab_bytes_, bytes);
+        PIS_HIP(hipMalloc((void**)&s.base, s.size));
+        if (s.size > bytes) s.free[bytes] = s.size - bytes;
+        *out = s.base;
+        reserved_ += s.size;
+        slabs_.push_back(std::move(s));
+        return PIS_OK;
+    }
+
+    void release(void* p, size_t bytes)
+    {
+        bytes = align256(bytes);
+        char* q = (char*)p;
+        for (auto& s : slabs_) {
+            if (q < s.base || q >= s.base + s.size) continue;
+            size_t off = (size_t)(q - s.base), len = bytes;
+            auto next = s.free.lower_bound(off);
+            if (next != s.free.end() && off + len == next->first) {  // merge with the range after
+                len += next->second;
+                next = s.free.erase(next);
+            }
+            if (next != s.free.begin()) {  // and with the range before
+                auto prev = std::prev(next);
+                if (prev->first + prev->second == off) {
+                    prev->second += len;
+                    return;
+                }
+            }
+            s.free[off] = len;
+            return;
+        }
+    }
+
+    size_t reserved() const { return reserved_; }
+
+private:
+    struct Slab {
+        char* base = nullptr;
+        size_t size = 0;
+        std::map<size_t, size_t> free;  // offset -> length
+    };
+    std::vector<Slab> slabs_;
+    size_t slab_bytes_;
+    size_t reserved_ = 0;
+};
+
 }  // namespace
 
 struct pis_server {
-    pis_ctx* ctx = nullptr;
+    pis_ctx* ctx = nullptr;   // running batch
+    pis_ctx* actx = nullptr;  // admission
+    pis_ctx* rctx = nullptr;  // collection
     std::vector<std::shared_ptr<Job>> resident;  // parallel to ctx->sys
+    std::vector<std::shared_ptr<Job>> admitted;  // parallel to actx->sys
+    std::vector<std::shared_ptr<Job>> retiring;  // parallel to rctx->sys
     std::deque<std::shared_ptr<Job>> pending;
     std::map<uint64_t, std::shared_ptr<Job>> jobs;  // submitted and not yet collected
-    std::mutex mu;
+    std::unique_ptr<DevicePool> pool;
+    std::mutex mu;  // guards the queue, jobs, budget and statistics
     std::condition_variable cv_work, cv_done;
     std::thread worker;
     bool stop = false;
@@ -62,6 +147,8 @@ struct pis_server {
     uint64_t submitted = 0, completed = 0, failed = 0, rounds = 0, atom_steps = 0, peak_bytes = 0;
     uint32_t peak_resident = 0;
     size_t n_resident = 0;  // resident.size() as last published under the lock
+    uint64_t graph_inst = 0, pool_bytes = 0;
+    double graph_s = 0.0;
     pis_status fatal = PIS_OK;
     std::string fatal_msg;
 
@@ -88,66 +175,94 @@ void finish(pis_server* sv, const std::shared_ptr<Job>& j, pis_status st, const 
     sv->cv_done.notify_all();
 }
 
-// Puts an admitted job on the device: its system (appended), state, list and initial
-// forces (a force launch over this system's tiles only), and its step budget.
-pis_status admit(pis_server* sv, const std::shared_ptr<Job>& j)
+// Gives a job's pool block back and takes it off the budget; called with the lock held
+// once no launch can touch the block any more.
+void release_block(pis_server* sv, Job& j)
 {
-    pis_ctx* c = sv->ctx;
-    PIS_TRY(ctx_add(c, Plan(j->plan)));
-    const uint32_t k = (uint32_t)c->sys.size() - 1;
-    c->sys[k].dev.dt = j->dt;
-    c->sys[k].dev.dt2 = j->dt * j->dt;  // dt.powi(2)
-    PIS_TRY(ensure_thermo(c, (int)sv->round_steps));
-    PIS_TRY(upload_sys(c));
-    PIS_TRY(sys_set_state(c, k, j->pos.data(), j->vel.data(), nullptr));
-    // staging growth in sys_set_state may have regrouped the tiles: find this system's
-    // tiles (its class lists them last, systems being grouped in index order)
-    const Plan& p = c->sys[k].plan;
-    for (const auto& q : c->classes) {
-        if (q.scap != p.scap || q.kind != plan_kind(p)) continue;
-        launch_force(c->stream, c->d_sys, c->d_tile_work + q.first + (q.count - p.ntile), p.ntile, q.scap,
-                     c->mixed, 0, plan_kind(p));
-    }
+    if (j.mem) sv->pool->release(j.mem, j.bytes);
+    j.mem = nullptr;
+    sv->used -= j.bytes;
+}
+
+std::vector<size_t> all_of(const pis_ctx* c)
+{
+    std::vector<size_t> v(c->sys.size());
+    for (size_t k = 0; k < v.size(); ++k) v[k] = k;
+    return v;
+}
+
+// Places one queued job in the admission context: its block of the pool, the arrays and
+// tables, the state (sorted and listed) and its step budget.  Its initial forces come
+// with the admission context's next force launch (admit_forces).
+pis_status admit_one(pis_server* sv, const std::shared_ptr<Job>& j)
+{
+    pis_ctx* a = sv->actx;
+    PIS_TRY(sv->pool->alloc(j->bytes, &j->mem));
+    std::vector<SysHost> add(1);
+    add[0].plan = j->plan;
+    PIS_TRY(alloc_system_at(add[0], j->mem, (int)sv->round_steps, a->stream));
+    add[0].dev.dt = j->dt;
+    add[0].dev.dt2 = j->dt * j->dt;  // dt.powi(2)
+    const pis_status put = ctx_put(a, std::move(add));
+    sv->admitted.push_back(j);  // the system is in the context now, whatever put returned
+    PIS_TRY(put);
+    const uint32_t k = (uint32_t)a->sys.size() - 1;
+    PIS_TRY(sys_set_state(a, k, j->pos.data(), j->vel.data(), nullptr));
     int32_t ctl[CTL_COUNT];
-    PIS_TRY(read_ctl(c, k, ctl));
+    PIS_TRY(read_ctl(a, k, ctl));
     ctl[CTL_LEFT] = (int32_t)j->nsteps;
     ctl[CTL_STEP] = 0;
-    PIS_TRY(write_ctl(c, k, ctl));
+    return write_ctl(a, k, ctl);
+}
+
+// Initial forces of every admitted system (one force launch per class of the admission
+// context, integrate off), waited for.
+pis_status admit_forces(pis_server* sv)
+{
+    pis_ctx* a = sv->actx;
+    if (a->sys.empty()) return PIS_OK;
+    PIS_TRY(enqueue_classes(a, false, 0));
     PIS_HIP(hipGetLastError());
+    PIS_HIP(hipStreamSynchronize(a->stream));
     return PIS_OK;
 }
 
-// Copies a finished job's final state and last thermo record to the host.
-pis_status collect(pis_server* sv, uint32_t k, const int32_t* ctl, Job& j)
+// The collection context's jobs: final state and last thermo record to the host, then
+// the jobs end and their blocks go back to the pool.
+pis_status collect_all(pis_server* sv)
 {
-    pis_ctx* c = sv->ctx;
-    const size_t n = (size_t)c->sys[k].plan.n;
-    j.out_pos.resize(3 * n);
-    j.out_vel.resize(3 * n);
-    j.out_force.resize(3 * n);
-    PIS_TRY(sys_get_state(c, k, j.out_pos.data(), j.out_vel.data(), j.out_force.data()));
-    const int rec = ctl[CTL_STEP] - 1;
-    if (rec >= 0 && rec < c->sys[k].thermo_cap) {
-        ThermoDev t;
-        PIS_HIP(hipMemcpy(&t, c->sys[k].thermo + rec, sizeof(ThermoDev), hipMemcpyDeviceToHost));
-        j.last = pis_thermo{t.pe, t.ke, t.etot, t.temperature, t.pressure, t.virial_abs, t.virial_pair};
+    pis_ctx* r = sv->rctx;
+    if (r->sys.empty()) return PIS_OK;
+    for (size_t k = 0; k < r->sys.size(); ++k) {
+        Job& j = *sv->retiring[k];
+        const size_t n = (size_t)r->sys[k].plan.n;
+        j.out_pos.resize(3 * n);
+        j.out_vel.resize(3 * n);
+        j.out_force.resize(3 * n);
+        PIS_TRY(sys_get_state(r, (uint32_t)k, j.out_pos.data(), j.out_vel.data(), j.out_force.data()));
+        int32_t ctl[CTL_COUNT];
+        PIS_TRY(read_ctl(r, (uint32_t)k, ctl));
+        const int rec = ctl[CTL_STEP] - 1;
+        if (rec >= 0 && rec < r->sys[k].thermo_cap) {
+            ThermoDev t;
+            PIS_HIP(hipMemcpyAsync(&t, r->sys[k].thermo + rec, sizeof(ThermoDev), hipMemcpyDeviceToHost, r->stream));
+            PIS_HIP(hipStreamSynchronize(r->stream));
+            j.last = pis_thermo{t.pe, t.ke, t.etot, t.temperature, t.pressure, t.virial_abs, t.virial_pair};
+        }
     }
+    std::vector<SysHost> gone;
+    PIS_TRY(ctx_take(r, all_of(r), &gone));
+    std::lock_guard<std::mutex> lk(sv->mu);
+    for (auto& j : sv->retiring) {
+        sv->atom_steps += (uint64_t)j->plan.n * j->nsteps;
+        release_block(sv, *j);
+        finish(sv, j, PIS_OK, "");
+    }
+    sv->retiring.clear();
     return PIS_OK;
 }
 
-// One round: `steps` batched NVE steps of every resident system (frozen ones skipped by
-// the kernels), replayed from a captured graph.
-pis_status run_round(pis_server* sv, int steps)
-{
-    pis_ctx* c = sv->ctx;
-    const int ns = (int)c->sys.size();
-    launch_set_ctl(c->stream, c->d_sys, ns, CTL_STEP, 0);
-    PIS_TRY(enqueue_steps(c, (uint64_t)steps, true, false));
-    PIS_HIP(hipGetLastError());
-    c->steps += (uint64_t)steps;
-    return PIS_OK;
-}
-
+// Control words of every resident system, in one gather launch and one copy.
 pis_status gather_ctl(pis_server* sv, std::vector<int32_t>& ctl)
 {
     pis_ctx* c = sv->ctx;
@@ -155,8 +270,9 @@ pis_status gather_ctl(pis_server* sv, std::vector<int32_t>& ctl)
     if (sv->d_ctl_cap < need) {
         if (sv->d_ctl) PIS_HIP(hipFree(sv->d_ctl));
         sv->d_ctl = nullptr;
-        PIS_HIP(hipMalloc(&sv->d_ctl, need * 4));
-        sv->d_ctl_cap = need;
+        sv->d_ctl_cap = 0;
+        PIS_HIP(hipMalloc(&sv->d_ctl, 2 * need * 4));
+        sv->d_ctl_cap = 2 * need;
     }
     launch_gather_ctl(c->stream, c->d_sys, (int)c->sys.size(), sv->d_ctl);
     ctl.resize(need);
@@ -165,109 +281,163 @@ pis_status gather_ctl(pis_server* sv, std::vector<int32_t>& ctl)
     return PIS_OK;
 }
 
-// Retires finished or failed resident jobs; returns their count.
-pis_status retire(pis_server* sv, int* nretired)
+// After a round (its stream idle): jobs whose budget ran out move to the collection
+// context; failed ones (capacity, non-finite values) end with their error at once.
+pis_status retire(pis_server* sv)
 {
     pis_ctx* c = sv->ctx;
-    *nretired = 0;
     if (c->sys.empty()) return PIS_OK;
     std::vector<int32_t> ctl;
     PIS_TRY(gather_ctl(sv, ctl));
-    std::vector<size_t> gone;
+    struct Out {
+        size_t k;
+        pis_status st;
+        std::string msg;
+    };
+    std::vector<Out> out;
     for (size_t k = 0; k < c->sys.size(); ++k) {
         const int32_t* q = &ctl[k * CTL_COUNT];
-        auto& j = sv->resident[k];
-        pis_status st = PIS_OK;
-        std::string msg;
-        if (q[CTL_OVF_STAGE]) {
-            st = PIS_ERR_CAPACITY;
-            msg = "LDS staging capacity exceeded";
-        } else if (q[CTL_OVF_NB]) {
-            st = PIS_ERR_CAPACITY;
-            msg = "neighbour capacity " + std::to_string(c->sys[k].plan.maxnb) + " exceeded";
-        } else if (q[CTL_NAN]) {
-            st = PIS_ERR_NAN_ENERGY;
-            msg = "non-finite energy or position";
-        } else if (q[CTL_LEFT] != 0) {
-            continue;  // still running
-        }
-        if (st == PIS_OK) {
-            pis_status cs = collect(sv, (uint32_t)k, q, *j);
-            if (cs != PIS_OK) {
-                st = cs;
-                msg = last_error();
+        if (q[CTL_OVF_STAGE])
+            out.push_back({k, PIS_ERR_CAPACITY, "LDS staging capacity exceeded"});
+        else if (q[CTL_OVF_NB])
+            out.push_back({k, PIS_ERR_CAPACITY, "neighbour capacity " + std::to_string(c->sys[k].plan.maxnb) +
+                                                    " exceeded (" + std::to_string(q[CTL_MAXNB]) + " needed)"});
+        else if (q[CTL_NAN])
+            out.push_back({k, PIS_ERR_NAN_ENERGY, "non-finite energy or position"});
+        else if (q[CTL_LEFT] == 0)
+            out.push_back({k, PIS_OK, ""});
+    }
+    if (out.empty()) return PIS_OK;
+    std::vector<size_t> idx;
+    for (const auto& o : out) idx.push_back(o.k);
+    std::vector<SysHost> taken;
+    PIS_TRY(ctx_take(c, idx, &taken));  // in ascending index order, like `out`
+    std::vector<SysHost> to_collect;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        for (size_t q = 0; q < out.size(); ++q) {
+            auto& j = sv->resident[out[q].k];
+            if (out[q].st != PIS_OK) {
+                release_block(sv, *j);
+                finish(sv, j, out[q].st, out[q].msg);
+            } else {
+                to_collect.push_back(std::move(taken[q]));
+                sv->retiring.push_back(j);
             }
         }
-        gone.push_back(k);
-        std::lock_guard<std::mutex> lk(sv->mu);
-        sv->used -= j->bytes;
-        if (st == PIS_OK) sv->atom_steps += (uint64_t)c->sys[k].plan.n * j->nsteps;
-        finish(sv, j, st, msg);
     }
-    if (!gone.empty()) {
-        PIS_TRY(ctx_remove(c, gone));
-        for (size_t g = gone.size(); g-- > 0;) sv->resident.erase(sv->resident.begin() + (ptrdiff_t)gone[g]);
-    }
-    *nretired = (int)gone.size();
-    return PIS_OK;
+    for (size_t q = out.size(); q-- > 0;) sv->resident.erase(sv->resident.begin() + (ptrdiff_t)out[q].k);
+    return ctx_put(sv->rctx, std::move(to_collect));
+}
+
+// The next queued job if it fits the slots and the budget (arrival order: a head that does
+// not fit holds back those behind it), taken off the queue and charged; null otherwise.
+std::shared_ptr<Job> next_admissible(pis_server* sv)
+{
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (sv->stop || sv->pending.empty()) return nullptr;
+    if (sv->resident.size() + sv->admitted.size() >= sv->max_systems) return nullptr;
+    auto j = sv->pending.front();
+    if (sv->used + j->bytes > sv->budget) return nullptr;
+    sv->pending.pop_front();
+    sv->used += j->bytes;
+    sv->peak_bytes = std::max(sv->peak_bytes, sv->used);
+    return j;
+}
+
+// A device-level failure ends every job and stops the server.
+void fail_all(pis_server* sv, pis_status st)
+{
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->fatal = st;
+    sv->fatal_msg = last_error();
+    for (auto* v : {&sv->resident, &sv->admitted, &sv->retiring})
+        for (auto& j : *v)
+            if (!j->done) finish(sv, j, st, sv->fatal_msg);
+    for (auto& j : sv->pending) finish(sv, j, st, sv->fatal_msg);
+    sv->resident.clear();
+    sv->admitted.clear();
+    sv->retiring.clear();
+    sv->pending.clear();
+    sv->n_resident = 0;
+    sv->stop = true;
 }
 
 void worker_main(pis_server* sv)
 {
     (void)hipSetDevice(sv->ctx->device);
     for (;;) {
-        std::vector<std::shared_ptr<Job>> admit_now;
         {
             std::unique_lock<std::mutex> lk(sv->mu);
-            sv->cv_work.wait(lk, [&] { return sv->stop || !sv->pending.empty() || !sv->resident.empty(); });
+            sv->cv_work.wait(lk, [&] {
+                return sv->stop || !sv->pending.empty() || !sv->resident.empty() || !sv->retiring.empty();
+            });
             if (sv->stop) break;
-            // arrival order: the head waits until it fits, later jobs wait behind it
-            while (!sv->pending.empty() && sv->resident.size() + admit_now.size() < sv->max_systems &&
-                   sv->used + sv->pending.front()->bytes <= sv->budget) {
-                admit_now.push_back(sv->pending.front());
-                sv->used += sv->pending.front()->bytes;
-                sv->pending.pop_front();
-            }
-            sv->peak_bytes = std::max(sv->peak_bytes, sv->used);
         }
-        for (auto& j : admit_now) {
+        pis_status st = PIS_OK;
+        // a round of the resident batch on its stream
+        const bool running = !sv->resident.empty();
+        if (running) {
+            pis_ctx* c = sv->ctx;
+            launch_set_ctl(c->stream, c->d_sys, (int)c->sys.size(), CTL_STEP, 0);
+            st = enqueue_steps(c, (uint64_t)sv->round_steps, true, false);
+            if (st == PIS_OK) {
+                const hipError_t e = hipGetLastError();
+                if (e != hipSuccess) st = fail(PIS_ERR_HIP, std::string("round launch: ") + hipGetErrorString(e));
+            }
+            c->steps += sv->round_steps;
+        }
+        // meanwhile, on the other streams: the last round's finished jobs leave the device
+        // and queued jobs come on, until the round is over (at least one admission a round)
+        if (st == PIS_OK) st = collect_all(sv);
+        while (st == PIS_OK) {
+            auto j = next_admissible(sv);
+            if (!j) break;
             j->t_admit = sv->now();
-            sv->resident.push_back(j);
-            pis_status st = admit(sv, j);
-            if (st != PIS_OK) {
+            const pis_status as = admit_one(sv, j);
+            if (as != PIS_OK) {
+                // the job fails alone: its system (if it got that far) leaves the
+                // admission context, whose stream is idle after the failed call
                 const std::string msg = last_error();
-                // drop the half-admitted system (if it was appended)
-                if (sv->ctx->sys.size() == sv->resident.size()) (void)ctx_remove(sv->ctx, {sv->resident.size() - 1});
-                sv->resident.pop_back();
+                if (!sv->admitted.empty() && sv->admitted.back() == j) {
+                    std::vector<SysHost> gone;
+                    st = ctx_take(sv->actx, {sv->actx->sys.size() - 1}, &gone);
+                    sv->admitted.pop_back();
+                }
                 std::lock_guard<std::mutex> lk(sv->mu);
-                sv->used -= j->bytes;
-                finish(sv, j, st, msg);
+                release_block(sv, *j);
+                finish(sv, j, as, msg);
+                continue;
+            }
+            if (running && hipStreamQuery(sv->ctx->stream) != hipErrorNotReady) break;
+        }
+        if (st == PIS_OK) st = admit_forces(sv);
+        // end of the round: finished jobs out, admitted ones in
+        if (st == PIS_OK && running) {
+            const hipError_t e = hipStreamSynchronize(sv->ctx->stream);
+            if (e != hipSuccess) st = fail(PIS_ERR_HIP, std::string("round: ") + hipGetErrorString(e));
+            if (st == PIS_OK) st = retire(sv);
+        }
+        if (st == PIS_OK && !sv->admitted.empty()) {
+            std::vector<SysHost> moving;
+            st = ctx_take(sv->actx, all_of(sv->actx), &moving);
+            if (st == PIS_OK) st = ctx_put(sv->ctx, std::move(moving));
+            if (st == PIS_OK) {
+                sv->resident.insert(sv->resident.end(), sv->admitted.begin(), sv->admitted.end());
+                sv->admitted.clear();
             }
         }
-        {
-            std::lock_guard<std::mutex> lk(sv->mu);
-            sv->n_resident = sv->resident.size();
-            sv->peak_resident = std::max(sv->peak_resident, (uint32_t)sv->resident.size());
-        }
-        if (sv->resident.empty()) continue;
-        pis_status st = run_round(sv, (int)sv->round_steps);
-        int nret = 0;
-        if (st == PIS_OK) st = retire(sv, &nret);
         if (st != PIS_OK) {
-            // a device-level failure ends every resident and queued job
-            std::lock_guard<std::mutex> lk(sv->mu);
-            sv->fatal = st;
-            sv->fatal_msg = last_error();
-            for (auto& j : sv->resident) finish(sv, j, st, sv->fatal_msg);
-            for (auto& j : sv->pending) finish(sv, j, st, sv->fatal_msg);
-            sv->resident.clear();
-            sv->pending.clear();
-            sv->stop = true;
+            fail_all(sv, st);
             break;
         }
         std::lock_guard<std::mutex> lk(sv->mu);
-        ++sv->rounds;
+        if (running) ++sv->rounds;
         sv->n_resident = sv->resident.size();
+        sv->peak_resident = std::max(sv->peak_resident, (uint32_t)sv->resident.size());
+        sv->graph_inst = sv->ctx->graph_instantiations;
+        sv->graph_s = sv->ctx->graph_build_s;
+        sv->pool_bytes = sv->pool->reserved();
     }
 }
 
@@ -284,15 +454,28 @@ pis_status pis_server_create(uint64_t hbm_budget_bytes, uint32_t max_systems, ui
         return fail(PIS_ERR_INVALID_ARGUMENT, "need max_systems > 0, 0 < round_steps <= 100000, budget > 0");
     if (precision != PIS_PRECISION_FP64 && precision != PIS_PRECISION_MIXED)
         return fail(PIS_ERR_INVALID_ARGUMENT, "unknown precision");
-    pis_ctx* c = nullptr;
-    PIS_TRY(ctx_create_empty(precision == PIS_PRECISION_MIXED, &c));
+    const bool mixed = precision == PIS_PRECISION_MIXED;
+    pis_ctx *c = nullptr, *a = nullptr, *r = nullptr;
+    pis_status st = ctx_create_empty(mixed, &c);
+    if (st == PIS_OK) st = ctx_create_empty(mixed, &a);
+    if (st == PIS_OK) st = ctx_create_empty(mixed, &r);
+    if (st != PIS_OK) {
+        free_ctx(c);
+        free_ctx(a);
+        free_ctx(r);
+        return st;
+    }
     c->use_graph = true;
     pis_server* sv = new pis_server();
     sv->ctx = c;
+    sv->actx = a;
+    sv->rctx = r;
+    // slabs of 1/16 of the budget, at least 256 MiB: a handful of hipMalloc calls a run
+    sv->pool.reset(new DevicePool(std::max<size_t>((size_t)256 << 20, hbm_budget_bytes / 16)));
     sv->budget = hbm_budget_bytes;
     sv->max_systems = max_systems;
     sv->round_steps = round_steps;
-    sv->mixed = precision == PIS_PRECISION_MIXED;
+    sv->mixed = mixed;
     sv->t0 = Clock::now();
     sv->worker = std::thread(worker_main, sv);
     *out = sv;
@@ -319,9 +502,10 @@ pis_status pis_server_submit(pis_server* sv, const pis_system_desc* desc, const 
         j->vel.assign(3 * n, 0.0);
     j->dt = dt;
     j->nsteps = nsteps;
-    j->bytes = system_bytes(j->plan, (int)sv->round_steps);
+    j->bytes = align256(system_bytes(j->plan, (int)sv->round_steps));
     std::lock_guard<std::mutex> lk(sv->mu);
-    if (sv->stop) return fail(sv->fatal != PIS_OK ? sv->fatal : PIS_ERR_INVALID_ARGUMENT, "server stopped: " + sv->fatal_msg);
+    if (sv->stop)
+        return fail(sv->fatal != PIS_OK ? sv->fatal : PIS_ERR_INVALID_ARGUMENT, "server stopped: " + sv->fatal_msg);
     if (j->bytes > sv->budget)
         return fail(PIS_ERR_OUT_OF_MEMORY, "job needs " + std::to_string(j->bytes) + " bytes, above the server budget");
     j->id = sv->next_id++;
@@ -386,6 +570,9 @@ pis_status pis_server_get_stats(pis_server* sv, pis_server_stats* out)
     out->peak_bytes = sv->peak_bytes;
     out->peak_resident = sv->peak_resident;
     out->reserved = 0;
+    out->graph_instantiations = sv->graph_inst;
+    out->graph_build_s = sv->graph_s;
+    out->pool_bytes = sv->pool_bytes;
     return PIS_OK;
 }
 
@@ -402,11 +589,16 @@ pis_status pis_server_destroy(pis_server* sv)
     if (sv->worker.joinable()) sv->worker.join();
     {
         std::lock_guard<std::mutex> lk(sv->mu);
-        for (auto& j : sv->resident)
-            if (!j->done) finish(sv, j, PIS_ERR_INVALID_ARGUMENT, "server destroyed");
+        for (auto* v : {&sv->resident, &sv->admitted, &sv->retiring})
+            for (auto& j : *v)
+                if (!j->done) finish(sv, j, PIS_ERR_INVALID_ARGUMENT, "server destroyed");
     }
     if (sv->d_ctl) (void)hipFree(sv->d_ctl);
+    // the contexts first (they leave pooled systems' memory alone), then the pool's slabs
     free_ctx(sv->ctx);
+    free_ctx(sv->actx);
+    free_ctx(sv->rctx);
+    sv->pool.reset();
     delete sv;
     return PIS_OK;
 }

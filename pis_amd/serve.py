@@ -4,10 +4,13 @@
         jid = srv.submit(spec, nsteps=1000)          # returns at once
         res = srv.wait(jid)                          # final state + last-step thermo
 
-A native worker thread owns the device: it keeps every admitted job's state resident,
-advances all of them together in rounds of `round_steps` batched NVE steps, and between
-rounds retires finished jobs and admits queued ones (arrival order) while the HBM budget
-and the slot limit allow.  Each job's result equals the same job run alone.
+A native worker thread owns the device: it keeps every admitted job's state resident in
+one block of a device memory pool and advances all of them together in rounds of
+`round_steps` batched NVE steps.  While a round runs, on streams of their own, jobs that
+finished in the previous round leave the device and queued jobs (arrival order, while the
+HBM budget and the slot limit allow) are placed, listed and given their initial forces;
+between rounds they swap in and out of the running batch.  Each job's result equals the
+same job run alone.
 """
 from __future__ import annotations
 

@@ -11,7 +11,9 @@ import time
 import numpy as np
 import pytest
 
+from oracle import pyoracle as po
 from pis_amd import Engine, PisError, Server, workloads as W
+from tests.conftest import force_rms_err
 
 pytestmark = pytest.mark.gpu
 
@@ -119,3 +121,66 @@ def test_wait_timeout_then_collect():
         with pytest.raises(PisError) as e:
             srv.wait(jid)
         assert e.value.status == 11
+
+
+def test_large_job_among_small_ones_equals_alone_and_its_forces_match_the_oracle():
+    """A 131,072-atom liquid job served next to small jobs that come and go: its final
+    state is the job run alone, bit for bit (mixed precision), and the job's step-0
+    forces (the admission's force launch, the same kernels) are within the mixed-precision
+    bound of the oracle on a seeded sample of 4,000 atoms."""
+    big = W.lj_liquid(32, 0.8442, 0.72, seed=31)  # 4 * 32^3 = 131,072 atoms
+    big.steps = 60
+    small = jobs(6, seed=13)
+    with Server(hbm_budget_bytes=16 << 30, max_systems=8, round_steps=16, precision="mixed") as srv:
+        ids = [srv.submit(small[0]), srv.submit(small[1])]
+        big_id = srv.submit(big)
+        for s in small[2:]:
+            time.sleep(0.01)
+            ids.append(srv.submit(s))
+        r = srv.wait(big_id)
+        rs = [srv.wait(j, state=False) for j in ids]
+        st = srv.stats()
+    print(f"N={big.n}: latency {r.latency_s:.3f}s, stats {st}", flush=True)
+    assert r.status == 0, r.error
+    assert all(x.status == 0 for x in rs)
+    th, (pos, vel, frc) = alone(big, "mixed")
+    assert np.array_equal(r.positions, pos)
+    assert np.array_equal(r.velocities, vel)
+    assert np.array_equal(r.forces, frc)
+    assert np.array_equal(r.thermo, th)
+    with Engine(big, "mixed") as eng:
+        eng.load()
+        eng.compute_potential()
+        f0 = eng.get_state(0)[2]
+    idx = np.sort(np.random.default_rng(31).choice(big.n, 4000, replace=False))
+    f_ref, _ = po.OSystem.from_spec(big).forces_subset(idx)
+    err = force_rms_err(f0[idx], f_ref)
+    print(f"step-0 force RMS error vs oracle: {err:.3e}", flush=True)
+    assert err <= 1e-5
+
+
+def test_jobs_reusing_released_memory_equal_the_jobs_alone():
+    """Jobs of nearly one shape (one staging class, so every round has the same launch
+    topology) pass through three slots: each newly admitted job lands in pool memory that a
+    retired job of another size used, and the resident set changes at every round.  Every
+    job still equals the job run alone, and the round's step graph was captured again after
+    each change."""
+    rng = np.random.default_rng(61)
+    specs = []
+    for k in range(10):
+        s = W.lj_box(int(rng.integers(1400, 1700)), 0.85, 1.0, seed=70 + k)
+        s.steps = int(rng.integers(1, 5)) * 16
+        specs.append(s)
+    with Server(hbm_budget_bytes=8 << 30, max_systems=3, round_steps=16, precision="fp64") as srv:
+        ids = [srv.submit(s) for s in specs]
+        res = [srv.wait(j) for j in ids]
+        st = srv.stats()
+    print(f"stats {st}", flush=True)
+    assert all(r.status == 0 for r in res)
+    assert st["graph_instantiations"] >= 4
+    for s, r in zip(specs, res):
+        th, (pos, vel, frc) = alone(s, "fp64")
+        assert np.array_equal(r.positions, pos)
+        assert np.array_equal(r.velocities, vel)
+        assert np.array_equal(r.forces, frc)
+        assert np.array_equal(r.thermo, th)
