@@ -48,7 +48,7 @@
 #define PIS_BUILD_MIN_WAVES 1
 #endif
 #ifndef PIS_SORT_GRID_CAP
-#define PIS_SORT_GRID_CAP 16384
+#define PIS_SORT_GRID_CAP 1024
 #endif
 #ifndef PIS_BUILD_GRID_CAP
 #define PIS_BUILD_GRID_CAP (1 << 30)
@@ -825,13 +825,15 @@ __device__ __forceinline__ int wave_max(int v)
 // shift register that leaves as one uint4 list group per 8 entries.  r2 from the staged
 // coordinates decides every pair outside a +-band around the cutoff; inside the band the
 // bit-exact reference predicate (exact_within, raw positions) decides.
-template <int SCAP, bool PAIRMI, bool MULTI, bool TRI>
+template <int SCAP, bool PAIRMI, int NT, bool TRI>
 __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, const TileGeom& g,
                                           const float* Px, const float* Py, const float* Pz,
                                           const uint8_t* Pt, const int* pref, const int* ecell,
                                           const PairParam* ppl, int total, bool active, int a,
                                           double cx, double cy, double cz)
 {
+    // NT: 1 (one species), 2 (a binary mixture) or kMaxTypes (any count up to that)
+    constexpr bool MULTI = NT > 1;
     const int nt = s.ntypes;
     double xi = 0.0, yi = 0.0, zi = 0.0;
     int ti = 0, e0 = 0, self = -1, ly = 0, lz = 0;
@@ -948,7 +950,10 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
                         }
                         const float r2 = fmaf(dx[j], dx[j], fmaf(dy[j], dy[j], dz[j] * dz[j]));
                         float blo = tlo[0], bhi = thi[0];
-                        if (MULTI) {
+                        if (NT == 2) {  // one select per edge: the partner is type 0 or 1
+                            blo = tt[j] != 0 ? tlo[1] : blo;
+                            bhi = tt[j] != 0 ? thi[1] : bhi;
+                        } else if (MULTI) {
                             blo = tt[j] == 1 ? tlo[1] : blo;
                             bhi = tt[j] == 1 ? thi[1] : bhi;
                             blo = tt[j] == 2 ? tlo[2] : blo;
@@ -1103,10 +1108,12 @@ __global__ __launch_bounds__(kTileThreads, PIS_BUILD_MIN_WAVES) void k_build_lis
             const bool active = a < a1;
             // REL: tile-relative staging (plain differences); otherwise raw positions and the
             // per-pair image of the box
-            const int n = nt == 1 ? build_atom<SCAP, !REL, false, TRI>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl,
-                                                                       total, active, a, cx, cy, cz)
-                                  : build_atom<SCAP, !REL, true, TRI>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl,
-                                                                      total, active, a, cx, cy, cz);
+            const int n = nt == 1 ? build_atom<SCAP, !REL, 1, TRI>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl,
+                                                                   total, active, a, cx, cy, cz)
+                          : nt == 2 ? build_atom<SCAP, !REL, 2, TRI>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl,
+                                                                     total, active, a, cx, cy, cz)
+                                    : build_atom<SCAP, !REL, kMaxTypes, TRI>(s, st, g, Px, Py, Pz, Pt, pref, ecell,
+                                                                             ppl, total, active, a, cx, cy, cz);
             mymax = max(mymax, n);
         }
         mymax = wave_max(mymax);
