@@ -87,8 +87,10 @@ struct PairParam {
     double c24;       // 24 eps
     double c48;       // 48 eps
     double ucut;      // energy shift U(rcut) (0 without shift)
+    double isig;      // 1 / sigma: the single-species mixed loop stages coordinates in sigma units
     float sig2f, c4f, c24f, c48f, ucutf;
     float r2f;        // FP32 cutoff of the mixed pair loop (rcut^2 rounded to FP32)
+    float r2s32;      // the same cutoff in sigma units, (rcut / sigma)^2 rounded to FP32
     // list-band edges for r^2 from the list build's FP32 staging (wider: the staged
     // coordinates carry FP32 rounding); -1 for an absent pair
     float l_lo32, l_hi32;

@@ -282,6 +282,8 @@ pis_status make_plan_impl(const pis_system_desc* d, Plan* p, bool need_types)
         q.c48f = (float)q.c48;
         q.ucutf = (float)q.ucut;
         q.r2f = (float)(c.rcut * c.rcut);
+        q.isig = 1.0 / c.sigma;
+        q.r2s32 = (float)((c.rcut / c.sigma) * (c.rcut / c.sigma));
         p->pp[(size_t)a * nt + b] = q;
         p->pp[(size_t)b * nt + a] = q;
         p->rcmax = std::max(p->rcmax, c.rcut);

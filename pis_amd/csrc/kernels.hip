@@ -323,11 +323,12 @@ constexpr double kPadFar = 1e30;  // squared: 1e60, finite in FP64, +inf in FP32
 // Returns the number of staged atoms (uniform over the block); > SCAP means overflow
 // and nothing was staged.  smap (SCAP + 1 bytes, may be null) receives the halo cell of
 // every slot, each cell's thread filling its run, so a slot finds its atom with one LDS
-// read instead of a binary search over the halo's cell offsets.
+// read instead of a binary search over the halo's cell offsets.  Staged coordinates are
+// multiplied by `scale` (1, or 1/sigma for the single-species mixed loop).
 template <int SCAP, int WHAT, typename T>
 __device__ int stage_tile(const SysDev& s, const State& st, const TileGeom& g, double cx, double cy,
                           double cz, T* Px, T* Py, T* Pz, uint8_t* Pt, int* pref, int* ecell, int* stmp,
-                          uint8_t* smap = nullptr)
+                          uint8_t* smap = nullptr, double scale = 1.0)
 {
     const int tid = threadIdx.x;
     int cnt = 0;
@@ -380,9 +381,9 @@ __device__ int stage_tile(const SysDev& s, const State& st, const TileGeom& g, d
         } else {
             double x = gbl(st.x)[a], y = gbl(st.y)[a], z = gbl(st.z)[a];
             if (WHAT == STAGE_REL) rel_to_center(s, cx, cy, cz, x, y, z);
-            Px[sl] = (T)x;
-            Py[sl] = (T)y;
-            Pz[sl] = (T)z;
+            Px[sl] = (T)(x * scale);
+            Py[sl] = (T)(y * scale);
+            Pz[sl] = (T)(z * scale);
             if (Pt) Pt[sl] = gbl(st.type)[a];  // the force kernel reads types from the list
         }
     }
@@ -632,16 +633,35 @@ __global__ __launch_bounds__(1024) void k_scan(const SysDev* __restrict__ sys)
     }
     int total;
     int run = block_excl_scan(sum, tmp, &total);
-    for (int i = i0; i < i1; i += 4) {
+    const int* __restrict__ cc = s.cell_count;
+    int* __restrict__ cs = s.cell_start;
+    i = i0;
+    // again four int4 loads issued before their stores (the arrays do not overlap)
+    for (; i + 16 <= i1; i += 16) {
+        int4 c[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) c[u] = *reinterpret_cast<const int4*>(cc + i + 4 * u);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            int4 o;
+            o.x = run;
+            o.y = o.x + c[u].x;
+            o.z = o.y + c[u].y;
+            o.w = o.z + c[u].z;
+            run = o.w + c[u].w;
+            *reinterpret_cast<int4*>(cs + i + 4 * u) = o;
+        }
+    }
+    for (; i < i1; i += 4) {
         if (i + 4 <= i1) {
-            const int4 c = *reinterpret_cast<const int4*>(s.cell_count + i);
+            const int4 c = *reinterpret_cast<const int4*>(cc + i);
             int4 o;
             o.x = run;
             o.y = o.x + c.x;
             o.z = o.y + c.y;
             o.w = o.z + c.z;
             run = o.w + c.w;
-            *reinterpret_cast<int4*>(s.cell_start + i) = o;
+            *reinterpret_cast<int4*>(cs + i) = o;
         } else {
             for (int k = i; k < i1; ++k) {
                 s.cell_start[k] = run;
@@ -1203,6 +1223,7 @@ __device__ __forceinline__ void atom_forces(const SysDev& s, const double* Px, c
     const char* bz = reinterpret_cast<const char*>(Pz);
     u32x4 un = {0u, 0u, 0u, 0u};
     if (ng > 0) un = L[0];
+    uint32_t nin = 0;  // single species: pairs within the cutoff
     for (int gi = 0; gi < ng; ++gi) {
         const u32x4 u = un;
         if (gi + 1 < ng) un = L[(size_t)(gi + 1) * s.npad];
@@ -1271,13 +1292,29 @@ __device__ __forceinline__ void atom_forces(const SysDev& s, const double* Px, c
 
 // FP32 parameters of one type pair for the mixed pair loop with FP32 staging (32 bytes:
 // two LDS reads per list entry in mixtures).
+// The cutoff of the single-species FP32 loop in three VALU instructions: the compare lands
+// in an SGPR pair that selects r^2 or +inf for the reciprocal and is also the carry-in that
+// counts the pair into n (n += r2 <= b); written out, the count would take a select and an
+// add.  v_cmp_ge_f32(b, r2) is false for NaN.
+__device__ __forceinline__ float cut_and_count(float r2, float b, float inf, uint32_t& n)
+{
+    uint64_t m, c;
+    float sel;
+    asm("v_cmp_ge_f32_e64 %[m], %[b], %[r2]\n\t"
+        "v_cndmask_b32_e64 %[sel], %[inf], %[r2], %[m]\n\t"
+        "v_addc_co_u32_e64 %[n], %[c], 0, %[n], %[m]"
+        : [m] "=&s"(m), [sel] "=&v"(sel), [n] "+v"(n), [c] "=&s"(c)
+        : [b] "v"(b), [r2] "v"(r2), [inf] "v"(inf));
+    return sel;
+}
+
 struct PairF {
-    float r2, sig2, c4, c24, c48, ucut, pad0, pad1;
+    float r2, sig2, c4, c24, c48, ucut, r2s, pad1;
 };
 
 __device__ __forceinline__ PairF pair_f32(const PairParam& p)
 {
-    return PairF{p.r2f, p.sig2f, p.c4f, p.c24f, p.c48f, p.ucutf, 0.0f, 0.0f};
+    return PairF{p.r2f, p.sig2f, p.c4f, p.c24f, p.c48f, p.ucutf, p.r2s32, 0.0f};
 }
 
 // The mixed pair loop over one atom's list with FP32 tile-relative staging (SoA Px, Py, Pz
@@ -1300,6 +1337,7 @@ __device__ __forceinline__ void atom_forces_f32(const SysDev& s, const float* Px
     const char* bx = reinterpret_cast<const char*>(Px);
     u32x4 un = {0u, 0u, 0u, 0u};
     if (ng > 0) un = L[0];
+    uint32_t nin = 0;  // single species: pairs within the cutoff
     for (int gi = 0; gi < ng; ++gi) {
         const u32x4 u = un;
         if (gi + 1 < ng) un = L[(size_t)(gi + 1) * s.npad];
@@ -1313,6 +1351,32 @@ __device__ __forceinline__ void atom_forces_f32(const SysDev& s, const float* Px
             X[e] = *reinterpret_cast<const float*>(bx + ad);
             Y[e] = *reinterpret_cast<const float*>(bx + ad + YO);
             Z[e] = *reinterpret_cast<const float*>(bx + ad + ZO);
+        }
+        if constexpr (!MULTI) {
+            // one species, coordinates in sigma units: 1/r'^2 is (sigma/r)^2 itself; the
+            // energy and the pair virial follow from the sums of (sigma/r)^12 and (sigma/r)^6
+            // and the number of pairs in range (combined per atom after the loop), and the
+            // force sum is scaled back by 1/sigma there
+            float fx = 0.0f, fy = 0.0f, fz = 0.0f, s12 = 0.0f, s6 = 0.0f;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float dx = X[e] - qx, dy = Y[e] - qy, dz = Z[e] - qz;
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                const float sr2 = __builtin_amdgcn_rcpf(cut_and_count(r2, p0.r2s, __builtin_inff(), nin));
+                const float sr6 = sr2 * sr2 * sr2;
+                const float fp = fmaf(p0.c48, sr6, -p0.c24) * sr6 * sr2;
+                fx = fmaf(-fp, dx, fx);
+                fy = fmaf(-fp, dy, fy);
+                fz = fmaf(-fp, dz, fz);
+                s12 = fmaf(sr6, sr6, s12);
+                s6 += sr6;
+            }
+            acc.fx += (double)fx;
+            acc.fy += (double)fy;
+            acc.fz += (double)fz;
+            acc.pe += (double)s12;
+            acc.wp += (double)s6;
+            continue;
         }
         float fx = 0.0f, fy = 0.0f, fz = 0.0f, pe = 0.0f, wp = 0.0f;
 #pragma unroll
@@ -1339,6 +1403,16 @@ __device__ __forceinline__ void atom_forces_f32(const SysDev& s, const float* Px
         acc.fz += (double)fz;
         acc.pe += (double)pe;
         acc.wp += (double)wp;
+    }
+    if constexpr (!MULTI) {
+        // U = 4 eps ((s/r)^12 - (s/r)^6) - U(rc) and r . F = 48 eps (s/r)^12 - 24 eps (s/r)^6
+        const PairParam& pp = s.pp[0];
+        const double s12 = acc.pe, s6 = acc.wp;
+        acc.pe = pp.c4 * (s12 - s6) - pp.ucut * (double)nin;
+        acc.wp = pp.c48 * s12 - pp.c24 * s6;
+        acc.fx *= pp.isig;
+        acc.fy *= pp.isig;
+        acc.fz *= pp.isig;
     }
 }
 
@@ -1418,8 +1492,11 @@ __global__ __launch_bounds__(kTileThreads, (ForceWaves<MIXED, KIND>::value)) voi
         lds.ppl[q] = s.pp[q];
         lds.pf[q] = pair_f32(s.pp[q]);
     }
+    // the single-species FP32 loop works in sigma units (an absent pair keeps scale 1)
+    const double scale = (F32 && nt == 1 && s.pp[0].r2_force >= 0.0) ? s.pp[0].isig : 1.0;
     const int total = stage_tile<SCAP, REL ? STAGE_REL : STAGE_RAW, T>(
-        s, st, g, cx, cy, cz, lds.Px, lds.Py, lds.Pz, (uint8_t*)nullptr, lds.pref, lds.ecell, lds.stmp, lds.smap);
+        s, st, g, cx, cy, cz, lds.Px, lds.Py, lds.Pz, (uint8_t*)nullptr, lds.pref, lds.ecell, lds.stmp, lds.smap,
+        scale);
     double sum[4] = {0.0, 0.0, 0.0, 0.0};
     if (total <= SCAP) {
         const PairParam p0 = s.pp[0];  // uniform: scalar loads, no vector registers
@@ -1434,7 +1511,8 @@ __global__ __launch_bounds__(kTileThreads, (ForceWaves<MIXED, KIND>::value)) voi
             if constexpr (F32) {
                 const float* Px = reinterpret_cast<const float*>(lds.Px);
                 if (nt == 1)
-                    atom_forces_f32<SCAP, false>(s, Px, lds.pf, p0f, a, (float)qx, (float)qy, (float)qz, acc);
+                    atom_forces_f32<SCAP, false>(s, Px, lds.pf, p0f, a, (float)(qx * scale), (float)(qy * scale),
+                                                 (float)(qz * scale), acc);
                 else
                     atom_forces_f32<SCAP, true>(s, Px, lds.pf + ti * nt, p0f, a, (float)qx, (float)qy, (float)qz,
                                                 acc);
