@@ -64,7 +64,7 @@ int g_tile_override[3] = {0, 0, 0};
 // scratch (as compiled).
 double tile_lds(int scap, bool f32) { return (scap + 1) * (f32 ? 12.0 : 24.0) + 4800.0; }
 // Workgroups per CU the force kernel's registers allow (one wave per SIMD each).
-double force_wg_cap(bool f32) { return f32 ? 5.0 : 3.0; }
+double force_wg_cap(bool f32) { return f32 ? 4.0 : 3.0; }
 
 // Tile choice: maximise lane use of the 256-thread workgroup times an occupancy factor
 // (workgroups per CU the LDS allows, capped at the 3 the register file allows), with a

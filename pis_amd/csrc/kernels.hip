@@ -1442,7 +1442,7 @@ struct StageType<true> {
 // Registers per lane a force instantiation must fit in (the launch-bounds occupancy target):
 // the FP32-staged mixed kernel aims at PIS_FORCE_MIN_WAVES_F32 waves per SIMD.
 #ifndef PIS_FORCE_MIN_WAVES_F32
-#define PIS_FORCE_MIN_WAVES_F32 5
+#define PIS_FORCE_MIN_WAVES_F32 4
 #endif
 template <bool MIXED, int KIND>
 struct ForceWaves {
