@@ -20,7 +20,7 @@ constexpr int kTileThreads = 256;  // workgroup of the tile kernels (4 wave64)
 constexpr int kAtomThreads = 256;  // workgroup of the per-atom kernels
 constexpr int kMaxHalo = 256;      // halo-box cells a tile may stage
 constexpr int kMaxTypes = 4;
-constexpr int kMaxList = 512;      // neighbour-list capacity limit per atom
+constexpr int kMaxList = 4096;     // neighbour-list capacity limit per atom (list memory: 2 B per entry)
 constexpr int kMaxLanes = 4;       // streams a context spreads its staging classes over
 
 // ctl[] slots (device ints per system)

@@ -57,6 +57,8 @@ SPECS = {
     "ka_2048": lambda: W.ka_mixture(2048, 1.2, 1.0, seed=9),
     "ka_5000": lambda: W.ka_mixture(5000, 1.2, 0.7, seed=10),
     "lj_dilute_3000": lambda: W.lj_box(3000, 0.3, 1.5, seed=4),
+    # a long cutoff: about 520 list entries per atom, 4096-slot tiles of one cell
+    "lj_rc5_16384": lambda: W.lj_liquid(16, 0.8442, 1.0, seed=6, rcut=5.0),
 }
 
 
