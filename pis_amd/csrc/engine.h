@@ -126,6 +126,7 @@ struct pis_ctx {
     std::vector<hipEvent_t> join;
     hipEvent_t fork = nullptr;
     int nlanes = 1;
+    int max_lanes = pis::kMaxLanes;  // streams the classes may spread over (the first is `stream`)
     void* d_xfer = nullptr;
     size_t xfer_bytes = 0;
     // trajectory snapshots: positions copied on `stream`, sent to the host on snap_stream
@@ -153,6 +154,8 @@ pis_status upload_sys(pis_ctx* c);
 void free_ctx(pis_ctx* c);
 // Tiles of all systems grouped by staging class, one table upload; drops captured graphs.
 pis_status group_tiles(pis_ctx* c);
+// Side streams and events for `nlanes` lanes (created once, kept until the context ends).
+pis_status reserve_lanes(pis_ctx* c, int nlanes);
 pis_status ensure_xfer(pis_ctx* c, size_t bytes);
 pis_status read_ctl(pis_ctx* c, uint32_t k, int32_t* ctl);
 pis_status write_ctl(pis_ctx* c, uint32_t k, const int32_t* ctl);

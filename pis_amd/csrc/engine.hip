@@ -616,6 +616,20 @@ void pis::free_ctx(pis_ctx* c)
     delete c;
 }
 
+pis_status pis::reserve_lanes(pis_ctx* c, int nlanes)
+{
+    if (!c->fork && nlanes > 1) PIS_HIP(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
+    while ((int)c->side.size() + 1 < nlanes) {
+        hipStream_t x = nullptr;
+        hipEvent_t j = nullptr;
+        PIS_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+        c->side.push_back(x);
+        PIS_HIP(hipEventCreateWithFlags(&j, hipEventDisableTiming));
+        c->join.push_back(j);
+    }
+    return PIS_OK;
+}
+
 // Groups every tile of every system by its LDS staging class (one launch per class) and
 // uploads the work table.  Captured step graphs refer to the old table and are dropped.
 pis_status pis::group_tiles(pis_ctx* c)
@@ -635,7 +649,7 @@ pis_status pis::group_tiles(pis_ctx* c)
         c->classes.push_back({kv.first.first, kv.first.second, (int)tw.size(), (int)kv.second.size(), 0});
         tw.insert(tw.end(), kv.second.begin(), kv.second.end());
     }
-    // Classes share kMaxLanes streams (the device serves a few hardware queues per
+    // Classes share up to max_lanes streams (the device serves a few hardware queues per
     // process; more streams would only time-slice): largest estimated work first, each
     // onto the least loaded stream.
     {
@@ -643,7 +657,7 @@ pis_status pis::group_tiles(pis_ctx* c)
         for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
         auto work = [&](int k) { return (double)c->classes[k].count * c->classes[k].scap; };
         std::sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
-        const int nl = (int)std::min<size_t>(kMaxLanes, c->classes.size());
+        const int nl = (int)std::min<size_t>((size_t)std::max(1, c->max_lanes), c->classes.size());
         std::vector<double> load(nl, 0.0);
         for (int k : order) {
             const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
@@ -653,16 +667,7 @@ pis_status pis::group_tiles(pis_ctx* c)
         c->nlanes = nl;
     }
     for (auto& stale : c->graph_stale) stale = true;
-    if (!c->fork && c->nlanes > 1)
-        PIS_HIP(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
-    while ((int)c->side.size() + 1 < c->nlanes) {
-        hipStream_t x = nullptr;
-        hipEvent_t j = nullptr;
-        PIS_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
-        c->side.push_back(x);
-        PIS_HIP(hipEventCreateWithFlags(&j, hipEventDisableTiming));
-        c->join.push_back(j);
-    }
+    PIS_TRY(reserve_lanes(c, c->nlanes));
     c->n_tile_work = (int)tw.size();
     if (c->tile_work_cap < tw.size()) {
         // grow only (a free would synchronise the device): twice the need

@@ -466,6 +466,20 @@ pis_status pis_server_create(uint64_t hbm_budget_bytes, uint32_t max_systems, ui
         return st;
     }
     c->use_graph = true;
+    // The device serves a few hardware queues per process and streams share them in turn:
+    // four streams in all, created together -- the running batch on two lanes, admission
+    // and collection on one each -- so the round's launches do not queue behind admission
+    // or collection work.
+    c->max_lanes = 2;
+    a->max_lanes = 1;
+    r->max_lanes = 1;
+    st = reserve_lanes(c, 2);
+    if (st != PIS_OK) {
+        free_ctx(c);
+        free_ctx(a);
+        free_ctx(r);
+        return st;
+    }
     pis_server* sv = new pis_server();
     sv->ctx = c;
     sv->actx = a;
