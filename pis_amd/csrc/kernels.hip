@@ -44,6 +44,9 @@
 #ifndef PIS_BUILD_SHIFTIN
 #define PIS_BUILD_SHIFTIN 1
 #endif
+#ifndef PIS_BUILD_WINTAB  // 1: row windows from the sub-cell table; 0: binary search
+#define PIS_BUILD_WINTAB 1
+#endif
 #ifndef PIS_BUILD_MIN_WAVES  // waves per SIMD the list build's registers must allow
 #define PIS_BUILD_MIN_WAVES 1
 #endif
@@ -398,22 +401,23 @@ __device__ int stage_tile(const SysDev& s, const State& st, const TileGeom& g, d
     return total;
 }
 
-// Tile-local halo entry and LDS slot of own atom `a` (modes TILE_REL / RAW_MI).
-__device__ __forceinline__ void own_slot(const SysDev& s, const TileGeom& g, const int* pref, int a,
-                                         int* e0, int* slot, int* ly_out = nullptr, int* lz_out = nullptr)
+// Tile-local halo entry and LDS slot of own atom `a` (modes TILE_REL / RAW_MI); ownf[k] is
+// the first atom of the tile's own cell k (tile order, x fastest), staged in LDS.
+__device__ __forceinline__ void own_slot(const TileGeom& g, const int* pref, const int* ownf, int a, int* e0,
+                                         int* slot, int* ly_out = nullptr, int* lz_out = nullptr)
 {
-    int lo = g.c0, hi = g.c1 - 1;
+    int lo = 0, hi = g.sx * g.sy * g.sz - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (s.cell_start[mid] <= a)
+        if (ownf[mid] <= a)
             lo = mid;
         else
             hi = mid - 1;
     }
-    const int k = lo - g.c0;
+    const int k = lo;
     const int lx = k % g.sx, ly = (k / g.sx) % g.sy, lz = k / (g.sx * g.sy);
     *e0 = ((lz + 1) * (g.sy + 2) + (ly + 1)) * (g.sx + 2) + (lx + 1);
-    *slot = pref[*e0] + (a - s.cell_start[lo]);
+    *slot = pref[*e0] + (a - ownf[k]);
     if (ly_out) *ly_out = ly;
     if (lz_out) *lz_out = lz;
 }
@@ -836,21 +840,62 @@ __device__ __forceinline__ int wave_max(int v)
     return max(max(r0, r1), max(r2, r3));
 }
 
+// Window table of an orthorhombic tile-relative staging: each halo cell is cut into 8
+// equal slices along x, and wtab[8 e + k] is the first slot of cell e whose staged x lies
+// in slice k or above (wtab[8 e] = the cell's first slot; entry 8 ne = the staging's end).
+// Cells are x-sorted, so the 3-cell row starting at cell c reads its window bounds at
+// wtab[8 c + G] for G in [0, 24] without a search.
+constexpr int kWinSlices = 8;
+__device__ __forceinline__ int win_slice(float x, float face, float inv_h)
+{
+    return min(max((int)floorf((x - face) * inv_h), 0), kWinSlices - 1);
+}
+
+// Descending sort of the 9 row keys (window length << 16 | first slot): a 25-comparator
+// network (three sorted triples merged), every index static so the keys stay in registers.
+__device__ __forceinline__ void sort9_desc(uint32_t* k)
+{
+    constexpr int net[25][2] = {{0, 1}, {3, 4}, {6, 7}, {1, 2}, {4, 5}, {7, 8}, {0, 1}, {3, 4}, {6, 7},
+                                {0, 3}, {3, 6}, {0, 3}, {1, 4}, {4, 7}, {1, 4}, {2, 5}, {5, 8}, {2, 5},
+                                {1, 3}, {5, 7}, {2, 6}, {4, 6}, {2, 4}, {2, 3}, {5, 6}};
+#pragma unroll
+    for (int q = 0; q < 25; ++q) {
+        const uint32_t a = k[net[q][0]], b = k[net[q][1]];
+        k[net[q][0]] = max(a, b);
+        k[net[q][1]] = min(a, b);
+    }
+}
+
+typedef __attribute__((address_space(3))) const float lds_cf32;
+
+// An LDS address the compiler cannot relate to the other staging arrays, so the reads at
+// small constant offsets from it share one base register through the instruction's offset
+// field instead of taking an address add each.
+__device__ __forceinline__ lds_cf32* lds_opaque(const float* p)
+{
+    lds_cf32* q = (lds_cf32*)p;
+    asm volatile("" : "+v"(q));
+    return q;
+}
+
+
 // Full neighbour list at rcut_ij + skin, accepted by the predicate of build_neighbour_list
 // (lennard_jones.rs:398-405) decided exactly; entries are LDS byte offsets.
-// Neighbour list of the own atom held by this lane (lane per atom).  The lane sweeps its
-// cell's stencil -- 9 contiguous (dy,dz) rows of the staging -- in chunks of up to 64
-// candidates with a trip count that is uniform over the wave (the wave's longest chunk),
-// setting one bit per accepted candidate; the set bits are then appended to a 128-bit
+// Neighbour list of the own atom held by this lane (lane per atom).  The lane's stencil is
+// 9 contiguous (dy,dz) rows of the staging; each row is first cut to the x window that can
+// hold a partner, and the lane takes its rows longest first, so that the wave's lanes
+// sweep windows of similar length together (the trip count of a row is the wave's longest
+// window).  A row is swept in chunks of up to 64 candidates with a trip count uniform over
+// the wave, one bit per accepted candidate; the set bits are then appended to a 128-bit
 // shift register that leaves as one uint4 list group per 8 entries.  r2 from the staged
 // coordinates decides every pair outside a +-band around the cutoff; inside the band the
 // bit-exact reference predicate (exact_within, raw positions) decides.
 template <int SCAP, bool PAIRMI, int NT, bool TRI>
 __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, const TileGeom& g,
                                           const float* Px, const float* Py, const float* Pz,
-                                          const uint8_t* Pt, const int* pref, const int* ecell,
-                                          const PairParam* ppl, int total, bool active, int a,
-                                          double cx, double cy, double cz)
+                                          const uint8_t* Pt, const int* pref, const int* ecell, const int* ownf,
+                                          const PairParam* ppl, const uint16_t* wtab, int total, bool active, int a, double cx,
+                                          double cy, double cz)
 {
     // NT: 1 (one species), 2 (a binary mixture) or kMaxTypes (any count up to that)
     constexpr bool MULTI = NT > 1;
@@ -865,7 +910,7 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
         if (s.mode == MODE_WHOLE)
             self = a;
         else
-            own_slot(s, g, pref, a, &e0, &self, &ly, &lz);
+            own_slot(g, pref, ownf, a, &e0, &self, &ly, &lz);
     }
     double qx = xi, qy = yi, qz = zi;
     if (!PAIRMI) rel_to_center(s, cx, cy, cz, qx, qy, qz);
@@ -890,48 +935,86 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
         thi[t] = p.l_hi32;
         reach2 = fmax(reach2, p.l_hi);
     }
-    // Tile-relative staging of an orthorhombic box keeps each 3-cell row sorted by x, so a
-    // row shrinks to the x window that can hold a partner: half-width sqrt(reach^2 - dy^2 - dz^2), with dy, dz
-    // the atom's distances to the row's cells (0 in its own y or z layer).  Slack of 1e-9
-    // cell widths absorbs binning and rounding at the cell faces; the window widens by
-    // twice the FP32 error of the staged x.
-    const double slack = 1e-9 * fmax(s.cellw[0], fmax(s.cellw[1], s.cellw[2]));
-    const double ylo = (ly - 0.5 * g.sy) * s.cellw[1], zlo = (lz - 0.5 * g.sz) * s.cellw[2];
-    const double gym = fmax(0.0, (qy - ylo) - slack), gyp = fmax(0.0, (ylo + s.cellw[1] - qy) - slack);
-    const double gzm = fmax(0.0, (qz - zlo) - slack), gzp = fmax(0.0, (zlo + s.cellw[2] - qz) - slack);
+    // Row windows.  Tile-relative staging of an orthorhombic box keeps each 3-cell row
+    // sorted by x, so a row shrinks to the x window that can hold a partner: half-width
+    // sqrt(reach^2 - dy^2 - dz^2), with dy, dz the atom's distances to the row's cells (0 in
+    // its own y or z layer).  Slack of 1e-9 cell widths absorbs binning and rounding at the
+    // cell faces; the window widens by twice the FP32 error of the staged x.
+    const int nrows = s.mode == MODE_WHOLE ? 1 : 9;
+    uint32_t key[9];  // (window length << 16) | first slot; 0 for an empty row
+    {
+        const double slack = 1e-9 * fmax(s.cellw[0], fmax(s.cellw[1], s.cellw[2]));
+        const double ylo = (ly - 0.5 * g.sy) * s.cellw[1], zlo = (lz - 0.5 * g.sz) * s.cellw[2];
+        const double gym = fmax(0.0, (qy - ylo) - slack), gyp = fmax(0.0, (ylo + s.cellw[1] - qy) - slack);
+        const double gzm = fmax(0.0, (qz - zlo) - slack), gzp = fmax(0.0, (zlo + s.cellw[2] - qz) - slack);
+        const int hx = g.sx + 2, hxy = (g.sx + 2) * (g.sy + 2);
+        // window table: face of the first cell of the lane's rows, slices per unit x
+        const float cw32 = (float)s.cellw[0], inv_h = (float)kWinSlices / cw32;
+        const float f0 = ((float)(e0 % hx) - 2.0f - 0.5f * (float)g.sx) * cw32;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            int rb = 0, re = 0;
+            if (active && r < nrows) {
+                if (s.mode == MODE_WHOLE) {
+                    re = total;
+                } else {
+                    const int oy = (r % 3) - 1, oz = (r / 3) - 1;
+                    const int e = e0 + oy * hx + oz * hxy;
+                    rb = pref[e - 1];
+                    re = pref[e + 2];
+                    if (!PAIRMI && !TRI) {  // a tilted box's rows overlap in x: no window
+                        const double gy = oy < 0 ? gym : (oy > 0 ? gyp : 0.0);
+                        const double gz = oz < 0 ? gzm : (oz > 0 ? gzp : 0.0);
+                        const double w2 = reach2 - gy * gy - gz * gz;
+                        if (w2 < 0.0) {
+                            re = rb;
+                        } else {
+                            const double w = sqrt(w2) * (1.0 + 1e-9) + s.wslack;
+                            if (PIS_BUILD_WINTAB) {
+                                // slices of the row's first cell (face f0): the bounds move out
+                                // by 1 % of a slice, far beyond the FP32 rounding of x, w, f0
+                                const float w32 = (float)w * (1.0f + 1e-6f);
+                                const int gs = (int)floorf((qx32 - w32 - f0) * inv_h - 0.01f);
+                                const int ge = (int)floorf((qx32 + w32 - f0) * inv_h + 0.01f) + 1;
+                                const int t0 = (e - 1) * kWinSlices;
+                                rb = wtab[t0 + min(max(gs, 0), 3 * kWinSlices)];
+                                re = wtab[t0 + min(max(ge, 0), 3 * kWinSlices)];
+                            } else {
+                                const int wb = search_sorted<false>(Px, rb, re, qx - w);
+                                re = search_sorted<true>(Px, wb, re, qx + w);
+                                rb = wb;
+                            }
+                        }
+                    }
+                }
+            }
+            key[r] = re > rb ? ((uint32_t)(re - rb) << 16) | (uint32_t)rb : 0u;
+        }
+    }
+    if (nrows > 1) sort9_desc(key);
     uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;  // 8 list entries of 16 bits, w0 oldest
     const uint32_t lsh = (uint32_t)s.list_shift;
     int cnt = 0;
     const int maxg = s.maxnb >> 3;
     const size_t npad = s.npad;
     __attribute__((address_space(1))) u32x4* out = gbl(reinterpret_cast<u32x4*>(s.nbr + a));
-    const int nrows = s.mode == MODE_WHOLE ? 1 : 9;
-    const int hx = g.sx + 2, hxy = (g.sx + 2) * (g.sy + 2);
+    // One list entry: the LDS byte offset of slot sl's coordinate in the force kernel's
+    // staging (slot << list_shift: 8-byte FP64 or 4-byte FP32 slots), with the partner's
+    // type index in the low bits (mixtures), shifted into the register of the last 8; a
+    // full register leaves as one uint4 list group.
+    auto append = [&](uint32_t sl) {
+        w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
+        w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
+        w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
+        w3 = __builtin_amdgcn_alignbit((sl << lsh) | (MULTI ? (uint32_t)Pt[sl] : 0u), w3, 16);
+        ++cnt;
+        if ((cnt & 7) == 0 && (cnt >> 3) <= maxg) out[(size_t)((cnt >> 3) - 1) * npad] = u32x4{w0, w1, w2, w3};
+    };
     for (int r = 0; r < nrows; ++r) {
-        int rb = 0, re = 0;
-        if (active) {
-            if (s.mode == MODE_WHOLE) {
-                re = total;
-            } else {
-                const int oy = (r % 3) - 1, oz = (r / 3) - 1;
-                const int e = e0 + oy * hx + oz * hxy;
-                rb = pref[e - 1];
-                re = pref[e + 2];
-                if (!PAIRMI && !TRI) {  // a tilted box's rows overlap in x: no window
-                    const double gy = oy < 0 ? gym : (oy > 0 ? gyp : 0.0);
-                    const double gz = oz < 0 ? gzm : (oz > 0 ? gzp : 0.0);
-                    const double w2 = reach2 - gy * gy - gz * gz;
-                    if (w2 < 0.0) {
-                        re = rb;
-                    } else {
-                        const double w = sqrt(w2) * (1.0 + 1e-9) + s.wslack;
-                        const int wb = search_sorted<false>(Px, rb, re, qx - w);
-                        re = search_sorted<true>(Px, wb, re, qx + w);
-                        rb = wb;
-                    }
-                }
-            }
-        }
+        const int rb = (int)(key[0] & 0xffffu), re = rb + (int)(key[0] >> 16);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) key[q] = key[q + 1];
+        key[8] = 0u;
         // uniform over the wave by construction; readfirstlane makes the loops scalar
         const int rowmax = wave_max(re - rb);
         for (int cb = 0; cb < rowmax; cb += 64) {
@@ -943,6 +1026,9 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
             const int kend = (min(64, rowmax - cb) + U - 1) & ~(U - 1);
             const int sb = rb + cb;
             const int sr = sb < re ? sb : 0;
+            lds_cf32* px = lds_opaque(Px + sr);
+            lds_cf32* py = lds_opaque(Py + sr);
+            lds_cf32* pz = lds_opaque(Pz + sr);
             uint32_t mlo = 0, mhi = 0, blo_m = 0, bhi_m = 0;
             // two halves of 32 candidates: bit positions are scalar, no branch per candidate
             for (int half = 0; half < 2; ++half) {
@@ -953,11 +1039,10 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
                     int tt[U];
 #pragma unroll
                     for (int j = 0; j < U; ++j) {
-                        const int sli = sr + k + j;
-                        dx[j] = Px[sli] - qx32;
-                        dy[j] = Py[sli] - qy32;
-                        dz[j] = Pz[sli] - qz32;
-                        tt[j] = MULTI ? (int)Pt[sli] : 0;
+                        dx[j] = px[k + j] - qx32;
+                        dy[j] = py[k + j] - qy32;
+                        dz[j] = pz[k + j] - qz32;
+                        tt[j] = MULTI ? (int)Pt[sr + k + j] : 0;
                     }
 #pragma unroll
                     for (int j = 0; j < U; ++j) {
@@ -1038,21 +1123,11 @@ __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, cons
             // accepted slots, ascending, into the 8-entry shift register (w0 oldest)
             for (int half = 0; half < 2; ++half) {
                 uint32_t m = half ? mhi : mlo;
-                const uint32_t base = (uint32_t)(sb + 32 * half);
+                const uint32_t hbase = (uint32_t)(sb + 32 * half);
                 while (m) {
                     const uint32_t b = (uint32_t)__builtin_ctz(m);
                     m &= m - 1u;
-                    w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
-                    w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
-                    w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
-                    // entry: LDS byte offset of the slot's coordinate in the force kernel's
-                    // staging (slot << list_shift: 8-byte FP64 or 4-byte FP32 slots), with
-                    // the partner's type index in the low bits (mixtures)
-                    const uint32_t sl = base + b;
-                    w3 = __builtin_amdgcn_alignbit((sl << lsh) | (MULTI ? (uint32_t)Pt[sl] : 0u), w3, 16);
-                    ++cnt;
-                    if ((cnt & 7) == 0 && (cnt >> 3) <= maxg)
-                        out[(size_t)((cnt >> 3) - 1) * npad] = u32x4{w0, w1, w2, w3};
+                    append(hbase + b);
                 }
             }
         }
@@ -1098,6 +1173,8 @@ __global__ __launch_bounds__(kTileThreads, PIS_BUILD_MIN_WAVES) void k_build_lis
     __shared__ int ecell[kMaxHalo];
     __shared__ int stmp[16];
     __shared__ PairParam ppl[kMaxTypes * kMaxTypes];
+    __shared__ uint16_t wtab[(kMaxHalo + 1) * kWinSlices];  // row-window table (orthorhombic, tile-relative)
+    __shared__ int ownf[kMaxHalo];  // first atom of each own cell
     const int lane = threadIdx.x & 63;
     if (!block_rebuild_pending<true>(sys, tiles, ntw)) return;
     for (int b = blockIdx.x; b < ntw; b += gridDim.x) {
@@ -1119,6 +1196,40 @@ __global__ __launch_bounds__(kTileThreads, PIS_BUILD_MIN_WAVES) void k_build_lis
             __syncthreads();
             continue;
         }
+        if (s.mode != MODE_WHOLE) {
+            for (int k = threadIdx.x; k < g.sx * g.sy * g.sz; k += blockDim.x) {
+                const int lx = k % g.sx, ly = (k / g.sx) % g.sy, lz = k / (g.sx * g.sy);
+                ownf[k] = ecell[((lz + 1) * (g.sy + 2) + (ly + 1)) * (g.sx + 2) + (lx + 1)];
+            }
+        }
+        if (PIS_BUILD_WINTAB && KIND == KIND_ORTHO) {
+            // the row-window table, one entry per thread: a binary search of the cell's
+            // x-sorted slots for the first whose slice (same FP32 faces as the lookups) is
+            // >= k.  A search writes only its own entry, so the table is the same whatever
+            // the order of the threads, even in a cell whose staged x is not monotone (an
+            // atom at the image boundary of a small box staged on the far side, never a
+            // partner of the tile's own atoms).
+            const int hx = g.sx + 2;
+            const float cw32 = (float)s.cellw[0], inv_h = (float)kWinSlices / cw32;
+            for (int q = threadIdx.x; q < (g.ne + 1) * kWinSlices; q += blockDim.x) {
+                const int e = q / kWinSlices, k = q % kWinSlices;
+                int lo = e < g.ne ? pref[e] : total, hi = e < g.ne ? pref[e + 1] : total;
+                if (k > 0) {
+                    const float face = ((float)(e % hx) - 1.0f - 0.5f * (float)g.sx) * cw32;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (win_slice(Px[mid], face, inv_h) < k)
+                            lo = mid + 1;
+                        else
+                            hi = mid;
+                    }
+                }
+                wtab[q] = (uint16_t)lo;
+            }
+            __syncthreads();
+        } else {
+            __syncthreads();
+        }
         const int a0 = s.cell_start[g.c0], a1 = s.cell_start[g.c1];
         int mymax = 0;
         // whole waves iterate together (inactive lanes ride along) so the sweeps stay uniform
@@ -1128,12 +1239,12 @@ __global__ __launch_bounds__(kTileThreads, PIS_BUILD_MIN_WAVES) void k_build_lis
             const bool active = a < a1;
             // REL: tile-relative staging (plain differences); otherwise raw positions and the
             // per-pair image of the box
-            const int n = nt == 1 ? build_atom<SCAP, !REL, 1, TRI>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl,
+            const int n = nt == 1 ? build_atom<SCAP, !REL, 1, TRI>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ownf, ppl, wtab,
                                                                    total, active, a, cx, cy, cz)
-                          : nt == 2 ? build_atom<SCAP, !REL, 2, TRI>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ppl,
+                          : nt == 2 ? build_atom<SCAP, !REL, 2, TRI>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ownf, ppl, wtab,
                                                                      total, active, a, cx, cy, cz)
-                                    : build_atom<SCAP, !REL, kMaxTypes, TRI>(s, st, g, Px, Py, Pz, Pt, pref, ecell,
-                                                                             ppl, total, active, a, cx, cy, cz);
+                                    : build_atom<SCAP, !REL, kMaxTypes, TRI>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ownf,
+                                                                             ppl, wtab, total, active, a, cx, cy, cz);
             mymax = max(mymax, n);
         }
         mymax = wave_max(mymax);
