@@ -205,6 +205,28 @@ def test_repeat_runs_bit_identical():
         assert np.array_equal(a, b)
 
 
+def test_list_order_repeatable_in_a_box_four_cells_wide():
+    """A box four list cells wide is still staged tile-relative, so a halo cell can hold an
+    atom at the periodic image boundary that is staged on the far side of its cell (its x
+    out of order).  The list build's row windows must not depend on thread order there:
+    two builds give the same list entry for entry, the pair set of the oracle, and the
+    same 100-step trajectory."""
+    spec = W.lj_box(3000, 0.9, 1.2, seed=21, rcut=3.0)
+    out = []
+    for _ in range(2):
+        with Engine(spec, "fp64") as eng:
+            eng.load()
+            off, idx = eng.neighbor_list()
+            eng.compute_potential()
+            th = eng.run_nve(spec.dt, 100)
+            out.append((off, idx, th))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2], out[1][2])
+    st, ix = po.OSystem.from_spec(spec).list(rcut_add=spec.skin)
+    assert np.array_equal(po.csr_to_sorted_pairs(out[0][0], out[0][1]), po.csr_to_sorted_pairs(st, ix))
+
+
 def test_errors_map_to_status():
     spec = W.lj_liquid(4, 0.8442, 0.72, seed=1)
     bad = W.lj_liquid(4, 0.8442, 0.72, seed=1)
