@@ -1334,7 +1334,6 @@ __device__ __forceinline__ void atom_forces(const SysDev& s, const double* Px, c
     const char* bz = reinterpret_cast<const char*>(Pz);
     u32x4 un = {0u, 0u, 0u, 0u};
     if (ng > 0) un = L[0];
-    uint32_t nin = 0;  // single species: pairs within the cutoff
     for (int gi = 0; gi < ng; ++gi) {
         const u32x4 u = un;
         if (gi + 1 < ng) un = L[(size_t)(gi + 1) * s.npad];
@@ -1528,16 +1527,21 @@ __device__ __forceinline__ void atom_forces_f32(const SysDev& s, const float* Px
 }
 
 // LDS of a force-kernel workgroup; T is the staged coordinate type.
+// LDS of a force workgroup.  The FP32 staging reads only the FP32 parameter table; from
+// 3072 slots on it also stages without the slot-to-cell map (a search of the halo's cell
+// offsets per slot instead), which keeps a workgroup under 40 KB so that four share a CU.
 template <typename T, int SCAP>
 struct ForceLds {
+    static constexpr bool F32 = sizeof(T) == 4;
+    static constexpr bool MAP = !(F32 && SCAP >= 3072);
     T Px[SCAP + 1], Py[SCAP + 1], Pz[SCAP + 1];
     int pref[kMaxHalo + 1];
     int ecell[kMaxHalo];
     int stmp[16];
     double rtmp[4 * (kTileThreads / 64)];
-    PairParam ppl[kMaxTypes * kMaxTypes];
+    PairParam ppl[F32 ? 1 : kMaxTypes * kMaxTypes];
     PairF pf[kMaxTypes * kMaxTypes];
-    uint8_t smap[SCAP + 1];
+    uint8_t smap[MAP ? SCAP + 1 : 1];
 };
 
 template <bool F32>
@@ -1600,14 +1604,14 @@ __global__ __launch_bounds__(kTileThreads, (ForceWaves<MIXED, KIND>::value)) voi
     tile_center(s, g, cx, cy, cz);
     const int nt = s.ntypes;
     for (int q = threadIdx.x; q < nt * nt; q += blockDim.x) {
-        lds.ppl[q] = s.pp[q];
+        if constexpr (!F32) lds.ppl[q] = s.pp[q];
         lds.pf[q] = pair_f32(s.pp[q]);
     }
     // the single-species FP32 loop works in sigma units (an absent pair keeps scale 1)
     const double scale = (F32 && nt == 1 && s.pp[0].r2_force >= 0.0) ? s.pp[0].isig : 1.0;
     const int total = stage_tile<SCAP, REL ? STAGE_REL : STAGE_RAW, T>(
-        s, st, g, cx, cy, cz, lds.Px, lds.Py, lds.Pz, (uint8_t*)nullptr, lds.pref, lds.ecell, lds.stmp, lds.smap,
-        scale);
+        s, st, g, cx, cy, cz, lds.Px, lds.Py, lds.Pz, (uint8_t*)nullptr, lds.pref, lds.ecell, lds.stmp,
+        ForceLds<T, SCAP>::MAP ? lds.smap : (uint8_t*)nullptr, scale);
     double sum[4] = {0.0, 0.0, 0.0, 0.0};
     if (total <= SCAP) {
         const PairParam p0 = s.pp[0];  // uniform: scalar loads, no vector registers
