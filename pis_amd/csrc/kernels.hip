@@ -1526,10 +1526,10 @@ __device__ __forceinline__ void atom_forces_f32(const SysDev& s, const float* Px
     }
 }
 
-// LDS of a force-kernel workgroup; T is the staged coordinate type.
-// LDS of a force workgroup.  The FP32 staging reads only the FP32 parameter table; from
-// 3072 slots on it also stages without the slot-to-cell map (a search of the halo's cell
-// offsets per slot instead), which keeps a workgroup under 40 KB so that four share a CU.
+// LDS of a force workgroup; T is the staged coordinate type.  The FP32 staging reads only
+// the FP32 parameter table; from 3072 slots on it also stages without the slot-to-cell map
+// (a search of the halo's cell offsets per slot instead), which keeps a workgroup under
+// 40 KB so that four share a CU.
 template <typename T, int SCAP>
 struct ForceLds {
     static constexpr bool F32 = sizeof(T) == 4;
