@@ -893,9 +893,9 @@ __device__ __forceinline__ lds_cf32* lds_opaque(const float* p)
 template <int SCAP, bool PAIRMI, int NT, bool TRI>
 __device__ __forceinline__ int build_atom(const SysDev& s, const State& st, const TileGeom& g,
                                           const float* Px, const float* Py, const float* Pz,
-                                          const uint8_t* Pt, const int* pref, const int* ecell, const int* ownf,
-                                          const PairParam* ppl, const uint16_t* wtab, int total, bool active, int a, double cx,
-                                          double cy, double cz)
+                                          const uint8_t* Pt, const int* pref, const int* ecell,
+                                          const int* ownf, const PairParam* ppl, const uint16_t* wtab,
+                                          int total, bool active, int a, double cx, double cy, double cz)
 {
     // NT: 1 (one species), 2 (a binary mixture) or kMaxTypes (any count up to that)
     constexpr bool MULTI = NT > 1;
@@ -1239,12 +1239,13 @@ __global__ __launch_bounds__(kTileThreads, PIS_BUILD_MIN_WAVES) void k_build_lis
             const bool active = a < a1;
             // REL: tile-relative staging (plain differences); otherwise raw positions and the
             // per-pair image of the box
-            const int n = nt == 1 ? build_atom<SCAP, !REL, 1, TRI>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ownf, ppl, wtab,
-                                                                   total, active, a, cx, cy, cz)
-                          : nt == 2 ? build_atom<SCAP, !REL, 2, TRI>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ownf, ppl, wtab,
-                                                                     total, active, a, cx, cy, cz)
-                                    : build_atom<SCAP, !REL, kMaxTypes, TRI>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ownf,
-                                                                             ppl, wtab, total, active, a, cx, cy, cz);
+            const int n =
+                nt == 1   ? build_atom<SCAP, !REL, 1, TRI>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ownf, ppl, wtab,
+                                                           total, active, a, cx, cy, cz)
+                : nt == 2 ? build_atom<SCAP, !REL, 2, TRI>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ownf, ppl, wtab,
+                                                           total, active, a, cx, cy, cz)
+                          : build_atom<SCAP, !REL, kMaxTypes, TRI>(s, st, g, Px, Py, Pz, Pt, pref, ecell, ownf,
+                                                                   ppl, wtab, total, active, a, cx, cy, cz);
             mymax = max(mymax, n);
         }
         mymax = wave_max(mymax);
