@@ -59,7 +59,25 @@ SPECS = {
     "lj_dilute_3000": lambda: W.lj_box(3000, 0.3, 1.5, seed=4),
     # a long cutoff: about 520 list entries per atom, 4096-slot tiles of one cell
     "lj_rc5_16384": lambda: W.lj_liquid(16, 0.8442, 1.0, seed=6, rcut=5.0),
+    # an orthorhombic box of unequal sides: cells 3.36 x 2.94 x 2.84 sigma (4 x 8 x 13)
+    "lj_aniso_9856": lambda: _anisotropic_liquid(),
 }
+
+
+def _anisotropic_liquid():
+    """FCC 8 x 14 x 22 unit cells at rho* = 0.8442, jittered by 0.1 sigma: a box of
+    13.4 x 23.5 x 37.0 sigma whose list cells differ in width along each axis."""
+    rng = np.random.default_rng(88)
+    a = (4.0 / 0.8442) ** (1.0 / 3.0)
+    lengths = [8 * a, 14 * a, 22 * a]
+    pos = W.fcc_sites(8, 14, 22, a) + rng.uniform(-0.1, 0.1, (4 * 8 * 14 * 22, 3))
+    pos = W.wrap_positions(pos, lengths)
+    n = pos.shape[0]
+    return W.SystemSpec(positions=pos, type_ids=np.ones(n, dtype=np.uint32), masses=np.array([1.0]),
+                        pairs=[(1, 1, 1.0, 1.0, 2.5)], box=W.SimulationBox.from_lammps_data(
+                            0.0, lengths[0], 0.0, lengths[1], 0.0, lengths[2]),
+                        velocities=W.start_velocities(np.ones(n), 0.9, 1.0, rng), skin=0.3, kb=1.0, dt=0.005,
+                        steps=100, name="lj_aniso")
 
 
 def _jittered_argon():
