@@ -731,6 +731,24 @@ __global__ __launch_bounds__(kAtomThreads) void k_rank_permute(const SysDev* __r
             a = gbl(s.scat_src)[p];
             sc = gbl(s.atom_cell)[a];
         }
+        const int cur = __builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR]);
+        const State src = s.st[cur], dst = s.st[cur ^ 1];
+        // the atom's state is read before the cell's keys are staged, so that its loads are
+        // in flight behind the staging's
+        double x = 0.0, y = 0.0, z = 0.0, v[3] = {0.0, 0.0, 0.0}, f[3] = {0.0, 0.0, 0.0};
+        uint8_t ty = 0;
+        if (act) {
+            x = gbl(src.x)[a];
+            y = gbl(src.y)[a];
+            z = gbl(src.z)[a];
+            v[0] = gbl(src.vx)[a];
+            v[1] = gbl(src.vy)[a];
+            v[2] = gbl(src.vz)[a];
+            f[0] = gbl(src.fx)[a];
+            f[1] = gbl(src.fy)[a];
+            f[2] = gbl(src.fz)[a];
+            ty = gbl(src.type)[a];
+        }
         // slots are grouped by cell, so the block's cells are those of its first and last slot
         if (threadIdx.x == 0) range[0] = gbl(s.cell_start)[sc];
         if (p == last) range[1] = gbl(s.cell_start)[sc + 1];
@@ -744,11 +762,8 @@ __global__ __launch_bounds__(kAtomThreads) void k_rank_permute(const SysDev* __r
             }
         __syncthreads();
         if (act) {
-            const int cur = __builtin_amdgcn_readfirstlane(s.ctl[CTL_CUR]);
-            const State src = s.st[cur], dst = s.st[cur ^ 1];
             const int c0 = gbl(s.cell_start)[sc], c1 = gbl(s.cell_start)[sc + 1];
             const uint32_t key = gbl(s.scat_key)[p];
-            const double x = gbl(src.x)[a], y = gbl(src.y)[a], z = gbl(src.z)[a];
             int rank = 0;
             if (staged) {
                 for (int q = c0 - base; q < c1 - base; ++q) {
@@ -770,13 +785,13 @@ __global__ __launch_bounds__(kAtomThreads) void k_rank_permute(const SysDev* __r
             gbl(s.rx)[np] = x;
             gbl(s.ry)[np] = y;
             gbl(s.rz)[np] = z;
-            gbl(dst.vx)[np] = gbl(src.vx)[a];
-            gbl(dst.vy)[np] = gbl(src.vy)[a];
-            gbl(dst.vz)[np] = gbl(src.vz)[a];
-            gbl(dst.fx)[np] = gbl(src.fx)[a];
-            gbl(dst.fy)[np] = gbl(src.fy)[a];
-            gbl(dst.fz)[np] = gbl(src.fz)[a];
-            gbl(dst.type)[np] = gbl(src.type)[a];
+            gbl(dst.vx)[np] = v[0];
+            gbl(dst.vy)[np] = v[1];
+            gbl(dst.vz)[np] = v[2];
+            gbl(dst.fx)[np] = f[0];
+            gbl(dst.fy)[np] = f[1];
+            gbl(dst.fz)[np] = f[2];
+            gbl(dst.type)[np] = ty;
             gbl(dst.orig)[np] = key;
         }
         __syncthreads();  // the next item restages the keys
