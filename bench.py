@@ -18,6 +18,7 @@ import json
 import math
 import os
 import platform
+import subprocess
 import sys
 import time
 
@@ -310,11 +311,16 @@ def run_dump(spec, steps, cadence):
     return out
 
 
-def run_ensemble(steps, warmup, direct_steps, accuracy=False):
+def run_ensemble(steps, warmup, direct_steps, accuracy=False, tune=True):
     """Batched ensemble: 512 Kob-Andersen systems (256-32,768 atoms, log-uniform from seed 7,
     density 1.2, T* 0.5-1.5) in one context, mixed precision.  Each step is one replay of
     the captured step graph (one launch per kernel per LDS staging class over all systems);
-    the same batch with direct launches is timed beside it."""
+    the same batch with direct launches is timed beside it.  With `tune`, the tiles are the
+    autotuner's: every launch class with at least 5 % of the atoms is timed as one batch at
+    the candidate tiles around the planner's pick, and moves to another tile only if that
+    is 10 % faster (untimed set-up, reported beside)."""
+    import dataclasses
+
     from pis_amd import Engine, workloads as W
 
     specs = W.ka_ensemble(512, seed=7, nmin=256, nmax=32768, steps=steps)
@@ -322,6 +328,22 @@ def run_ensemble(steps, warmup, direct_steps, accuracy=False):
     dts = [s.dt for s in specs]
     out = {"n_systems": len(specs), "total_atoms": natoms, "steps": steps,
            "min_atoms": min(s.n for s in specs), "max_atoms": max(s.n for s in specs)}
+    if tune:
+        # the autotuner (pis_amd.tune) in a child process, on the same seeded ensemble
+        t0 = time.perf_counter()
+        r = subprocess.run([sys.executable, "-m", "pis_amd.tune", "--ka-ensemble", "512", "--seed", "7",
+                            "--nmin", "256", "--nmax", "32768"], capture_output=True, text=True, timeout=600,
+                           cwd=os.path.dirname(os.path.abspath(__file__)))
+        if r.returncode != 0:
+            raise RuntimeError("ensemble autotuner failed: " + r.stderr[-2000:])
+        tuned = json.loads(r.stdout.strip().splitlines()[-1])
+        tiles = [tuple(t) if t else None for t in tuned["tiles"]]
+        assert len(tiles) == len(specs)
+        specs = [dataclasses.replace(s, tile=t) for s, t in zip(specs, tiles)]
+        out["autotune"] = {"seconds": round(time.perf_counter() - t0, 2), "classes": tuned["classes"],
+                           "systems_retiled": sum(t is not None for t in tiles)}
+        log(f"ensemble autotune: {len(tuned['classes'])} classes timed in {out['autotune']['seconds']} s, "
+            f"{out['autotune']['systems_retiled']} systems off the model's tile")
     with Engine(specs, "mixed") as eng:
         eng.load()
         eng.compute_potential()
@@ -590,6 +612,8 @@ def main():
     ap.add_argument("--dump-cadence", type=int, default=20)
     ap.add_argument("--ensemble-steps", type=int, default=5000,
                     help="steps of the 512-system batched ensemble (0: skip)")
+    ap.add_argument("--no-ensemble-tune", action="store_true",
+                    help="run the ensemble on the planner's model tiles instead of the autotuner's")
     ap.add_argument("--serve-jobs", type=int, default=500,
                     help="jobs of each Poisson serving stream (0: skip)")
     ap.add_argument("--serve-low-rate", type=float, default=25.0,
@@ -664,7 +688,8 @@ def main():
     ensemble = None
     if args.ensemble_steps:
         log(f"batched ensemble: 512 KA systems, {args.ensemble_steps} steps, graph replay")
-        ensemble = run_ensemble(args.ensemble_steps, 100, min(args.ensemble_steps, 1000), accuracy=not args.no_cpu)
+        ensemble = run_ensemble(args.ensemble_steps, 100, min(args.ensemble_steps, 1000), accuracy=not args.no_cpu,
+                                tune=not args.no_ensemble_tune)
         log(f"ensemble: {ensemble['total_atoms']} atoms, {ensemble['value']:.4e} atom-steps/s "
             f"(direct launches {ensemble.get('direct_launch_value', 0):.4e})")
 

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define PIS_ABI_VERSION 2
+#define PIS_ABI_VERSION 3
 
 /* Status codes.  1-4 map to PisError variants (errors.rs:71-99). */
 typedef enum pis_status {
@@ -83,7 +83,9 @@ typedef struct pis_system_desc {
     uint8_t pbc[3];       /* all three must be 1 (from_lammps_data always sets them) */
     uint8_t shift;        /* LennardJones::shift; the shipped path always sets 1 */
     uint8_t precision;    /* pis_precision */
-    uint8_t reserved[3];
+    uint8_t tile[3];      /* cells per tile along x, y, z, as an autotuner picked them (each
+                             clamped to what the cell grid allows); all zero: the planner's
+                             model picks.  It outranks pis_set_tile_override. */
     double skin;          /* Verlet skin; list cutoff is rcut_ij + skin, rebuilt when an atom
                              moved more than skin/2 since the last build */
     double kb;            /* Boltzmann constant used for temperature (constants.rs:3 is

@@ -55,7 +55,7 @@ class SystemDesc(C.Structure):
         ("pbc", C.c_uint8 * 3),
         ("shift", C.c_uint8),
         ("precision", C.c_uint8),
-        ("reserved", C.c_uint8 * 3),
+        ("tile", C.c_uint8 * 3),
         ("skin", C.c_double),
         ("kb", C.c_double),
         ("neighbor_capacity", C.c_uint32),

@@ -33,6 +33,7 @@ struct Plan {
     int64_t n = 0;
     int ntypes = 0;
     int nc[3] = {1, 1, 1}, tile[3] = {1, 1, 1}, nt[3] = {1, 1, 1};
+    int tile_req[3] = {0, 0, 0};  // a caller's tile (pis_system_desc::tile); zeros: the model
     int ncell = 1, ntile = 1, mode = MODE_WHOLE, maxnb = 8, scap = 1024, npad = 64;
     double L[3] = {0, 0, 0}, Linv[3] = {0, 0, 0}, cellw[3] = {0, 0, 0};
     double rcmax = 0, rcl = 0, skin = 0, density = 0, volume = 0, kb = 1, wslack = 0;

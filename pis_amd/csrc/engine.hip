@@ -101,9 +101,11 @@ void choose_tiles(Plan& p)
                     bt[2] = tz;
                 }
             }
-    if (g_tile_override[0] > 0) {
+    const bool req = p.tile_req[0] > 0 && p.tile_req[1] > 0 && p.tile_req[2] > 0;
+    if (req || g_tile_override[0] > 0) {
         int ov[3];
-        for (int d = 0; d < 3; ++d) ov[d] = std::max(1, std::min(g_tile_override[d], lim[d]));
+        for (int d = 0; d < 3; ++d)
+            ov[d] = std::max(1, std::min(req ? p.tile_req[d] : g_tile_override[d], lim[d]));
         if ((ov[0] + 2) * (ov[1] + 2) * (ov[2] + 2) <= kMaxHalo)
             for (int d = 0; d < 3; ++d) bt[d] = ov[d];
     }
@@ -219,6 +221,7 @@ pis_status make_plan_impl(const pis_system_desc* d, Plan* p, bool need_types)
     if (!(d->kb > 0.0)) return fail(PIS_ERR_INVALID_ARGUMENT, "kb must be > 0");
     const int nt = (int)d->n_types;
     p->n = (int64_t)d->n_atoms;
+    for (int k = 0; k < 3; ++k) p->tile_req[k] = d->tile[k];
     p->ntypes = nt;
     p->skin = d->skin;
     p->kb = d->kb;

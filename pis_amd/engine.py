@@ -52,6 +52,9 @@ class _DescHolder:
         d.kb = float(spec.kb)
         d.neighbor_capacity = int(neighbor_capacity)
         d.flags = 0
+        if spec.tile is not None:
+            for k in range(3):
+                d.tile[k] = int(spec.tile[k])
         self.desc = d
 
 

@@ -37,6 +37,7 @@ class SystemSpec:
     steps: int = 1000
     name: str = "system"
     meta: dict = field(default_factory=dict)
+    tile: tuple | None = None  # cells per tile from an autotuner; None: the planner's model
 
     @property
     def n(self) -> int:

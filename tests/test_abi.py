@@ -40,7 +40,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_status_strings():
     lib = pis_amd.lib()
-    assert lib.pis_abi_version() == 2
+    assert lib.pis_abi_version() == 3
     for code in range(14):
         assert lib.pis_status_string(code) != b"unknown status"
 

@@ -146,3 +146,43 @@ def test_tile_autotuner_times_candidates_and_restores_the_model_pick():
     assert best in [r[0] for r in results] and len(results) >= 2
     assert all(r[2] > 0 for r in results)
     assert pis_amd.plan(spec, "mixed")["tile"] == model
+
+
+def test_ensemble_autotuner_covers_every_class_and_retiled_systems_match_the_oracle():
+    """autotune_ensemble times every launch class (min_share 0) at its candidate tiles and
+    returns one tile per system from those candidates; a batch planned with tiles other
+    than the model's (pis_system_desc::tile) still gives FP64 forces within 1e-10 of
+    LJVOffsetManager and the model's planner is left in force."""
+    import dataclasses
+
+    from pis_amd import tune
+
+    specs = W.ka_ensemble(6, seed=11, nmin=3000, nmax=12000, steps=50)  # 3,121-10,863 atoms
+    models = [pis_amd.plan(s, "mixed")["tile"] for s in specs]
+    tiles, report = tune.autotune_ensemble(specs, "mixed", steps=10, warmup=2, limit=3, min_share=0.0)
+    classes = tune.ensemble_classes(specs, "mixed")
+    assert len(report) == len(classes) and len(tiles) == len(specs)
+    for r in report:
+        cands = [tuple(t) for t, ms in r["ms_per_step"]]
+        assert r["model_tile"] in cands and r["tile"] in cands and all(ms > 0 for _, ms in r["ms_per_step"])
+    for t, m in zip(tiles, models):
+        assert t is None or t != m
+    assert [pis_amd.plan(s, "mixed")["tile"] for s in specs] == models
+    # every system on a candidate tile that is not the model's pick
+    retiled = []
+    for s in specs:
+        m = pis_amd.plan(s, "fp64")["tile"]
+        other = next(t for t, _ in tune.candidates(s, "fp64", limit=4) if t != m)
+        retiled.append(dataclasses.replace(s, tile=other))
+        assert pis_amd.plan(retiled[-1], "fp64")["tile"] == other
+    with Engine(retiled, "fp64") as eng:
+        eng.load()
+        pe = eng.compute_potential()
+        forces = [eng.get_state(k)[2] for k in range(len(retiled))]
+    for k, s in enumerate(retiled):
+        o = po.OSystem.from_spec(s)
+        pe_ref = o.compute(po.MGR_OFFSET)
+        print(f"retiled system {k}: N={s.n} tile={s.tile}", flush=True)
+        assert force_err(forces[k], o.force) <= 1e-10
+        assert abs(pe[k] - pe_ref) <= 1e-10 * abs(pe_ref)
+
