@@ -2199,17 +2199,18 @@ __global__ __launch_bounds__(1024) void k_reduce(const SysDev* __restrict__ sys,
 {
     __shared__ double rtmp[64];
     const SysDev& s = sys[blockIdx.x];
-    if (step && s.ctl[CTL_LEFT] == 0) return;  // frozen: no step was taken
-    double v[4];
-    sum_partials(s, v, rtmp);
-    if (threadIdx.x != 0) return;
-    const double pe = 0.5 * v[0], ke = v[1], wabs = v[2], wpair = 0.5 * v[3];
-    double ham = pe + ke;
-    // control words: all read first, then written (one round trip, not one per word)
+    // control words: all read first, then written (one round trip, not one per word), and
+    // read before the partial sums so that the two round trips overlap
     __attribute__((address_space(1))) int32_t* ctl = gbl(s.ctl);
     const int c_cur = ctl[CTL_CUR], c_rb = ctl[CTL_REBUILD], c_nrb = ctl[CTL_NREBUILD];
     const int c_step = ctl[CTL_STEP], c_next = ctl[CTL_REBUILD_NEXT];
     const int c_remain = ctl[CTL_REMAIN], c_left = ctl[CTL_LEFT];
+    double v[4];
+    sum_partials(s, v, rtmp);
+    if (step && c_left == 0) return;  // frozen: no step was taken (uniform over the block)
+    if (threadIdx.x != 0) return;
+    const double pe = 0.5 * v[0], ke = v[1], wabs = v[2], wpair = 0.5 * v[3];
+    double ham = pe + ke;
     if (step && s.nhc_on) {
         // end of an NVT step (potential.rs:51-54), H with the chain's energy
         // (simulation.rs:100-105; the chain coordinates eta never move, so its potential
