@@ -84,8 +84,9 @@ typedef struct pis_system_desc {
     uint8_t shift;        /* LennardJones::shift; the shipped path always sets 1 */
     uint8_t precision;    /* pis_precision */
     uint8_t tile[3];      /* cells per tile along x, y, z, as an autotuner picked them (each
-                             clamped to what the cell grid allows); all zero: the planner's
-                             model picks.  It outranks pis_set_tile_override. */
+                             clamped to ncell - 2; ignored if the tile's halo would exceed
+                             the staging tables); any zero: no request, the planner's model
+                             (or pis_set_tile_override) picks.  It outranks the override. */
     double skin;          /* Verlet skin; list cutoff is rcut_ij + skin, rebuilt when an atom
                              moved more than skin/2 since the last build */
     double kb;            /* Boltzmann constant used for temperature (constants.rs:3 is
